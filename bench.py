@@ -1,0 +1,157 @@
+"""Benchmark: cubes/s on the d4t table, persistent batched form vs plain one-cube-at-a-time.
+
+One step = one batched solve of the workload (4,096 seeded uniform random
+states on d4t, one persistent launch).  The d4t table is built on the GPU
+first (BFS kernels, not timed).  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "cuda-cube-solver_amd"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PROBE = 32  # one 32-byte table block per lookup
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = max(0, min(len(xs) - 1, int(round(q / 100.0 * len(xs) + 0.5)) - 1))  # nearest rank
+    return xs[k]
+
+
+def cpu_baseline(tb, cubes, seconds_budget: float):
+    """The oracle (C restatement of the reference's search) on the host cores,
+    over the same GPU-built table, on a bounded prefix of the workload."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    table = np.frombuffer(tb.read("block", 0, tb.nbytes("block")), dtype=np.uint8)
+    k = O.kind("d4t")
+    threads = min(os.cpu_count() or 1, 16)
+    n = threads * 2
+    arr = O.cubes([(s.cp, s.co, s.ep, s.eo) for s in cubes[:n]])
+    t = time.perf_counter()
+    res, bad = O.solve_many(k, table, 0, arr, n_threads=threads)
+    dt = time.perf_counter() - t
+    lengths = [r["length"] for r in res]
+    return {"value": round(n / dt, 3), "unit": "cubes/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of the workload's states, one cube per thread, {dt:.1f} s"}, lengths
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cubes", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--plain", type=int, default=128, help="cubes solved one at a time for the plain figure")
+    ap.add_argument("--table", default="d4t")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--prefix-len", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver shards by "
+              f"replicating the table per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from cuda_cube_solver_amd import _lib, states
+    from cuda_cube_solver_amd.solver import GpuSolver, Tables
+
+    dev = _lib.device_info()
+    log(f"device {dev.name.decode()} {dev.arch.decode()} {dev.compute_units} CUs")
+    t = time.perf_counter()
+    tb, counts = Tables.build(args.table, "block", log=log)
+    log(f"built {args.table} in {time.perf_counter() - t:.1f}s, depth counts {counts}")
+    cubes = states.uniform_states(args.cubes, args.seed)
+    solver = GpuSolver(table=tb, mode="batched", prefix_len=args.prefix_len, blocks_per_cu=args.blocks_per_cu)
+
+    for w in range(args.warmup):
+        solver.solve(cubes[: min(512, len(cubes))], mode="batched")
+        log(f"warmup {w + 1}/{args.warmup} done")
+    step_s, probes, kernel_ms, lat = [], 0, 0.0, []
+    lengths = None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        sols = solver.solve(cubes, mode="batched")
+        step_s.append(time.perf_counter() - t)
+        st = solver.last_stats
+        probes += st.probes
+        kernel_ms += st.kernel_ms
+        lat = [s.seconds * 1e3 for s in sols]
+        lengths = [len(s.moves) for s in sols]
+        log(f"step {k + 1}/{args.steps}: {step_s[-1]:.2f}s, {st.nodes / 1e9:.2f} G nodes, "
+            f"{st.probes / 1e9:.2f} G probes")
+    total_s = sum(step_s)
+    value = args.cubes * args.steps / total_s
+
+    # plain: one cube at a time, per-round launches (the reference's form at batch 1)
+    plain_ms = []
+    plain_len = []
+    for s in cubes[: args.plain]:
+        t = time.perf_counter()
+        sol = solver.solve([s], mode="plain")[0]
+        plain_ms.append((time.perf_counter() - t) * 1e3)
+        plain_len.append(len(sol.moves))
+    plain_value = len(plain_ms) / (sum(plain_ms) / 1e3)
+    same = plain_len == lengths[: len(plain_len)]
+    log(f"plain {len(plain_ms)} cubes: {plain_value:.2f} cubes/s, lengths identical to batched: {same}")
+
+    achieved = probes * BYTES_PER_PROBE / (kernel_ms / 1e3) / 1e9
+    line = {
+        "metric": "cubes/s (d4t, seeded random states) batched vs plain on 1 GPU; p50/p99 solve ms",
+        "value": round(value, 3),
+        "unit": "cubes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_s / args.steps * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/u32 integer",
+        "data": "synthetic: seeded uniform random cube states (splitmix64), d4t table built by the GPU BFS",
+        "config": {"workload": f"{args.cubes} seeded uniform random states (seed {args.seed}) on d4t, one "
+                               f"persistent batched launch per step", "table": args.table,
+                   "prefix_len": args.prefix_len},
+        "batched_p50_ms": round(pct(lat, 50), 2),
+        "batched_p99_ms": round(pct(lat, 99), 2),
+        "plain_cubes_per_s": round(plain_value, 3),
+        "plain_p50_ms": round(pct(plain_ms, 50), 2),
+        "plain_p99_ms": round(pct(plain_ms, 99), 2),
+        "plain_cubes": len(plain_ms),
+        "batched_over_plain": round(value / plain_value, 2),
+        "lengths_identical": same,
+        "length_histogram": {str(L): lengths.count(L) for L in sorted(set(lengths))},
+        "probes_per_cube": round(probes / (args.cubes * args.steps)),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    if not args.no_cpu:
+        cb, cpu_len = cpu_baseline(tb, cubes, 20.0)
+        if cpu_len != lengths[: len(cpu_len)]:
+            log("CPU baseline lengths differ from the GPU's")
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

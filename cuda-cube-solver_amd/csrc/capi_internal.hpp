@@ -1,0 +1,85 @@
+// Internals shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <string>
+
+#include "../../include/cubesolver.h"
+#include "cube_model.hpp"
+#include "search_types.hpp"
+
+namespace ccs {
+
+extern thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(CCS_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+inline double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+inline int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 0;
+        cus = p.multiProcessorCount;
+    }
+    return cus;
+}
+
+inline int kind_K(const Kind& k) { return k.is304 ? 0 : k.nec; }
+
+inline uint64_t format_blocks(const Kind& k, int format) {
+    uint64_t per = format == CCS_FORMAT_BLOCK ? 124u : 155u;
+    return (k.n_entries + per - 1) / per;
+}
+
+}  // namespace ccs
+
+// A kind's device constants plus its resident solving table(s).
+struct ccs_tables {
+    ccs::Kind kind;
+    int K = 0;
+    int format = -1;  // solving format used by probes and searches
+    ccs::CubeLUT lut{};
+    ccs::CubeLUT* d_lut = nullptr;
+    uint32_t* d_classidx = nullptr;
+    uint32_t* d_class_rep = nullptr;
+    uint16_t* d_class_stab = nullptr;
+    uint8_t* d_table[2] = {nullptr, nullptr};
+    uint64_t table_bytes[2] = {0, 0};
+    uint8_t* d_prefix[ccs::MAX_PLEN + 1] = {};
+    uint32_t prefix_n[ccs::MAX_PLEN + 1] = {};
+    hipStream_t stream = nullptr;
+
+    ~ccs_tables() {
+        if (stream) hipStreamSynchronize(stream);
+        hipFree(d_lut);
+        hipFree(d_classidx);
+        hipFree(d_class_rep);
+        hipFree(d_class_stab);
+        for (auto* p : d_table) hipFree(p);
+        for (auto* p : d_prefix) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace ccs {
+int tables_setup(ccs_tables* t, const std::string& name);
+int ensure_prefixes(ccs_tables* t, int plen);
+}  // namespace ccs

@@ -1,0 +1,230 @@
+// Packed-cube arithmetic for CDNA4: moves, conjugation, table index, unrank
+// and table lookups, all integer VALU + LDS work.
+//
+// Functions are __host__ __device__ so that the library's diagnostic entry
+// ccs_index_host can evaluate exactly this LUT logic on the CPU; kernels are
+// the only product users.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lut.hpp"
+
+#define CCS_HD __host__ __device__ __forceinline__
+
+namespace ccs {
+
+struct Frame {
+    uint32_t c0, c1, e0, e1, e2;
+};
+
+constexpr uint32_t SOLVED_C0 = 0x03020100u, SOLVED_C1 = 0x07060504u;
+constexpr uint32_t SOLVED_E0 = 0x03020100u, SOLVED_E1 = 0x07060504u, SOLVED_E2 = 0x0B0A0908u;
+
+CCS_HD Frame solved_frame() { return Frame{SOLVED_C0, SOLVED_C1, SOLVED_E0, SOLVED_E1, SOLVED_E2}; }
+CCS_HD bool is_solved(const Frame& f) {
+    return ((f.c0 ^ SOLVED_C0) | (f.c1 ^ SOLVED_C1) | (f.e0 ^ SOLVED_E0) | (f.e1 ^ SOLVED_E1) | (f.e2 ^ SOLVED_E2)) == 0;
+}
+
+// v_perm_b32: bytes of {s0:s1} picked by selector bytes (see lut.hpp).
+CCS_HD uint32_t vperm(uint32_t s0, uint32_t s1, uint32_t sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(s0, s1, sel);
+#else
+    uint64_t data = ((uint64_t)s0 << 32) | s1;
+    uint32_t r = 0;
+    for (int i = 0; i < 4; ++i) {
+        uint32_t s = (sel >> (8 * i)) & 0xFFu, b;
+        if (s >= 13) b = 0xFFu;
+        else if (s == 12) b = 0;
+        else if (s >= 8) b = ((data >> (16 * (s - 8) + 15)) & 1u) ? 0xFFu : 0u;
+        else b = (uint32_t)(data >> (8 * s)) & 0xFFu;
+        r |= b << (8 * i);
+    }
+    return r;
+#endif
+}
+
+CCS_HD uint32_t popc(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+CCS_HD uint32_t ctz(uint32_t x) { return (uint32_t)__builtin_ctz(x); }
+CCS_HD uint32_t shl(uint32_t x, uint32_t s) { return s >= 32u ? 0u : x << s; }
+CCS_HD uint64_t shl64(uint64_t x, uint32_t s) { return s >= 64u ? 0u : x << s; }
+
+// Bit `bit` of each byte -> 4-bit mask (byte 0 -> bit 0).
+CCS_HD uint32_t gather4(uint32_t v, int bit) { return (((v >> bit) & 0x01010101u) * 0x01020408u) >> 24; }
+
+// Corner twist stored in bits 3..5 of each byte may reach 4 after an add: reduce mod 3.
+CCS_HD uint32_t twist_mod3(uint32_t x) { return x - (((x + 0x08080808u) >> 5) & 0x01010101u) * 0x18u; }
+
+// Relabel edge cubies of a word through a 12-byte table (rel0..rel2): new
+// byte = rel[cubie] with the existing flip bit xored in.
+CCS_HD uint32_t relabel_edges(uint32_t v, uint32_t r0, uint32_t r1, uint32_t r2) {
+    uint32_t sel = v & 0x0F0F0F0Fu;
+    uint32_t x = vperm(r1, r0, sel);                 // cubies 0..7 (8..11 read as zero)
+    uint32_t y = vperm(r2, r2, sel & 0x03030303u);   // cubies 8..11
+    uint32_t hi = ((v >> 3) & 0x01010101u) * 0xFFu;
+    return (x | (y & hi)) ^ (v & 0x10101010u);
+}
+
+// state then move m (right multiplication; kernels.cu:84-101 computes the same).
+CCS_HD void apply_move(const CubeLUT& L, Frame& f, int m) {
+    uint32_t a = twist_mod3(vperm(f.c1, f.c0, L.mc_sel[m][0]) + L.mc_add[m][0]);
+    uint32_t b = twist_mod3(vperm(f.c1, f.c0, L.mc_sel[m][1]) + L.mc_add[m][1]);
+    uint32_t e0 = (vperm(f.e1, f.e0, L.me_lo[m][0]) | vperm(f.e2, f.e2, L.me_hi[m][0])) ^ L.me_flip[m][0];
+    uint32_t e1 = (vperm(f.e1, f.e0, L.me_lo[m][1]) | vperm(f.e2, f.e2, L.me_hi[m][1])) ^ L.me_flip[m][1];
+    uint32_t e2 = (vperm(f.e1, f.e0, L.me_lo[m][2]) | vperm(f.e2, f.e2, L.me_hi[m][2])) ^ L.me_flip[m][2];
+    f = Frame{a, b, e0, e1, e2};
+}
+
+// move m then state (left multiplication; used by the inverse frames).
+CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) {
+    uint32_t a = twist_mod3(vperm(L.lc[m][1], L.lc[m][0], f.c0 & 0x07070707u) + (f.c0 & 0x18181818u));
+    uint32_t b = twist_mod3(vperm(L.lc[m][1], L.lc[m][0], f.c1 & 0x07070707u) + (f.c1 & 0x18181818u));
+    uint32_t e0 = relabel_edges(f.e0, L.le[m][0], L.le[m][1], L.le[m][2]);
+    uint32_t e1 = relabel_edges(f.e1, L.le[m][0], L.le[m][1], L.le[m][2]);
+    uint32_t e2 = relabel_edges(f.e2, L.le[m][0], L.le[m][1], L.le[m][2]);
+    f = Frame{a, b, e0, e1, e2};
+}
+
+CCS_HD int inverse_move(int m) { return 3 * (m / 3) + 2 - m % 3; }
+
+// twist (3^7) * 70 + rank of the D-corner positions (coords.py:112-118).
+CCS_HD uint32_t corner_raw(const CubeLUT& L, const Frame& f) {
+    uint32_t olo = (f.c0 >> 3) & 0x03030303u, ohi = (f.c1 >> 3) & 0x00030303u;
+    uint32_t twist = ((olo * 0x0103091Bu) >> 24) + 81u * ((ohi * 0x01030900u) >> 24);
+    uint32_t dmask = gather4(f.c0, 2) | (gather4(f.c1, 2) << 4);
+    return twist * 70u + L.ccomb_rank[dmask];
+}
+
+// K = 0: the 304 kind; K > 0: a group kind with K E-content classes.
+template <int K>
+struct KindT {
+    static constexpr uint64_t N_EDGE = K == 0 ? 495ull * 70ull * 16ull : 495ull * 256ull * (uint64_t)K;
+};
+
+// Edge coordinate of M_s g M_s^-1 (coords.py:121-154 for 304; kernels.cu:175-186
+// for the group kinds), from the packed edges.
+template <int K>
+CCS_HD uint32_t edge_coord(const CubeLUT& L, const Frame& f, uint32_t s) {
+    uint32_t e[3];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        uint32_t x = (vperm(f.e1, f.e0, L.cs_lo[s][w]) | vperm(f.e2, f.e2, L.cs_hi[s][w])) ^ L.cs_flip[s][w];
+        e[w] = relabel_edges(x, L.cs_rel[s][0], L.cs_rel[s][1], L.cs_rel[s][2]);
+    }
+    uint32_t m0 = gather4(e[0], 3), m1 = gather4(e[1], 3), m2 = gather4(e[2], 3);
+    uint32_t emask = m0 | (m1 << 4) | (m2 << 8);
+    uint32_t lo6 = emask & 63u;
+    uint32_t slice = (uint32_t)L.slice_lo[lo6] + L.slice_hi[popc(lo6)][emask >> 6];
+    uint32_t n0 = popc(m0), n1 = popc(m1);
+    // E bytes in position order, compacted into one word
+    uint32_t ew = vperm(e[0], e[0], L.csel[m0]) | shl(vperm(e[1], e[1], L.csel[m1]), 8u * n0) |
+                  shl(vperm(e[2], e[2], L.csel[m2]), 8u * (n0 + n1));
+    uint32_t eflip = gather4(ew, 4);
+    // non-E bytes in position order: 8 bytes in two words
+    uint32_t k0 = 4u - n0, k1 = 4u - n1;
+    uint64_t nw = (uint64_t)vperm(e[0], e[0], L.csel[m0 ^ 15u]) |
+                  shl64(vperm(e[1], e[1], L.csel[m1 ^ 15u]), 8u * k0) |
+                  shl64(vperm(e[2], e[2], L.csel[m2 ^ 15u]), 8u * (k0 + k1));
+    uint32_t nlo = (uint32_t)nw, nhi = (uint32_t)(nw >> 32);
+    if (K == 0) {
+        // U-layer edges are cubies 0..3 (bits 2 and 3 clear)
+        uint32_t umask = gather4(~(nlo | (nlo >> 1)), 2) | (gather4(~(nhi | (nhi >> 1)), 2) << 4);
+        return (slice * 70u + L.ccomb_rank[umask]) * 16u + eflip;
+    } else {
+        uint32_t lab = ew & 0x03030303u;
+        uint32_t key = (lab | (lab >> 6) | (lab >> 12) | (lab >> 18)) & 0xFFu;
+        uint32_t nflip = gather4(nlo, 4) | (gather4(nhi, 4) << 4);
+        uint32_t c = L.gen_class[(uint32_t)L.eorder_rank[key] * 16u + eflip];
+        return (slice * 256u + nflip) * (uint32_t)K + c;
+    }
+}
+
+// Table index: corner class * N_EDGE + edge coordinate after the class
+// symmetry (coords.py:347-352).
+template <int K>
+CCS_HD uint64_t table_index(const CubeLUT& L, const uint32_t* __restrict__ classidx, const Frame& f) {
+    uint32_t packed = classidx[corner_raw(L, f)];
+    return (uint64_t)(packed >> 4) * KindT<K>::N_EDGE + edge_coord<K>(L, f, packed & 15u);
+}
+
+// Lower bound from the solving table (kernels.cu:396-434 defines the formats):
+// F = 0: block (u8 minimum + 124 two-bit offsets per 32 B), F = 1: base 3
+// (header m | c << 6, 155 digits).
+template <int F>
+CCS_HD uint32_t table_lb(const uint8_t* __restrict__ tbl, uint64_t idx) {
+    if (F == 0) {
+        uint64_t b = idx / 124u;
+        uint32_t pos = 8u + 2u * (uint32_t)(idx - b * 124u);
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(tbl + b * 32u);
+        return (p[0] & 0xFFu) + ((p[pos >> 5] >> (pos & 31u)) & 3u);
+    } else {
+        uint64_t b = idx / 155u;
+        uint32_t j = (uint32_t)(idx - b * 155u);
+        const uint8_t* p = tbl + b * 32u;
+        uint32_t hdr = p[0], byte = p[1u + j / 5u], k = j % 5u;
+        uint32_t d = (byte / (k == 0 ? 1u : k == 1 ? 3u : k == 2 ? 9u : k == 3 ? 27u : 81u)) % 3u;
+        uint32_t m = hdr & 63u;
+        return d == 0u ? m : m + (hdr >> 6) + d;
+    }
+}
+
+// Representative frame of a table entry (the BFS expands it); any member of
+// the coset works since moves act on cosets.
+template <int K>
+CCS_HD Frame unrank(const CubeLUT& L, const uint32_t* __restrict__ class_rep, uint64_t idx) {
+    uint64_t cls = idx / KindT<K>::N_EDGE;
+    uint32_t ec = (uint32_t)(idx - cls * KindT<K>::N_EDGE);
+    uint32_t raw = class_rep[cls];
+    uint32_t twist = raw / 70u, dmask = L.ccomb_unrank[raw - twist * 70u];
+    uint32_t c[2] = {0, 0}, osum = 0, nu = 0, nd = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint32_t o;
+        if (i < 7) {
+            o = twist % 3u;
+            twist /= 3u;
+            osum += o;
+        } else {
+            o = (3u - osum % 3u) % 3u;
+        }
+        uint32_t cp = ((dmask >> i) & 1u) ? 4u + nd++ : nu++;
+        c[i >> 2] |= (cp | (o << 3)) << ((i & 3) * 8);
+    }
+    uint32_t emask, eord, umask, eflips, nflips;
+    if (K == 0) {
+        uint32_t eo = ec % 16u, rest = ec / 16u;
+        emask = L.slice_unrank[rest / 70u];
+        umask = L.ccomb_unrank[rest % 70u];
+        eord = 0xE4u;
+        eflips = eo;
+        nflips = 0;
+    } else {
+        uint32_t gc = ec % (uint32_t)K, grest = ec / (uint32_t)K;
+        nflips = grest % 256u;
+        emask = L.slice_unrank[grest / 256u];
+        uint32_t content = L.gen_rep[popc(nflips) & 1u][gc];
+        eord = L.eorder_unrank[content >> 4];
+        eflips = content & 15u;
+        umask = 0x0Fu;
+    }
+    uint32_t e[3] = {0, 0, 0}, ne = 0, no = 0, nue = 0, nde = 0;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        uint32_t b, fl;
+        if ((emask >> i) & 1u) {
+            b = 8u | ((eord >> (2 * ne)) & 3u);
+            fl = (eflips >> ne) & 1u;
+            ++ne;
+        } else {
+            b = ((umask >> no) & 1u) ? nue++ : 4u + nde++;
+            fl = (nflips >> no) & 1u;
+            ++no;
+        }
+        e[i >> 2] |= (b | (fl << 4)) << ((i & 3) * 8);
+    }
+    return Frame{c[0], c[1], e[0], e[1], e[2]};
+}
+
+}  // namespace ccs

@@ -1,0 +1,957 @@
+// CDNA4 (gfx950) kernels of the optimal cube solver: probes, the BFS that
+// builds the pruning tables, the table encoders and the IDA* search.
+// All integer work on the vector ALUs; lookup tables live in LDS.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "kernels.hpp"
+
+namespace ccs {
+
+namespace {
+
+__device__ __forceinline__ void load_lut(CubeLUT* dst, const CubeLUT* __restrict__ src) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    for (uint32_t i = threadIdx.x; i < sizeof(CubeLUT) / 16; i += blockDim.x) d[i] = s[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    uint32_t l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ uint32_t first_lane_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long first_lane_u64(unsigned long long v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
+template <typename T>
+__device__ __forceinline__ T ld_acquire(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_relaxed(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_release(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// probes
+
+template <int K, int F>
+__global__ void __launch_bounds__(256) probe_kernel(const CubeLUT* __restrict__ gL, const uint32_t* __restrict__ classidx,
+                                                    const uint8_t* __restrict__ table, const Frame* __restrict__ in,
+                                                    int n, int move, unsigned long long* __restrict__ idx_out,
+                                                    uint32_t* __restrict__ lb_out, Frame* __restrict__ state_out) {
+    __shared__ CubeLUT L;
+    load_lut(&L, gL);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Frame f = in[i];
+    if (move >= 0) apply_move(L, f, move);
+    if (move <= -2) apply_left(L, f, -2 - move);
+    uint64_t idx = table_index<K>(L, classidx, f);
+    if (idx_out) idx_out[i] = idx;
+    if (lb_out && table) lb_out[i] = table_lb<F>(table, idx);
+    if (state_out) state_out[i] = f;
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) unrank_kernel(const CubeLUT* __restrict__ gL, const uint32_t* __restrict__ classidx,
+                                                     const uint32_t* __restrict__ class_rep,
+                                                     const unsigned long long* __restrict__ idx, int n,
+                                                     unsigned long long* __restrict__ out) {
+    __shared__ CubeLUT L;
+    load_lut(&L, gL);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Frame f = unrank<K>(L, class_rep, idx[i]);
+    out[i] = table_index<K>(L, classidx, f);
+}
+
+// ---------------------------------------------------------------------------
+// BFS over the coset space.  The build table holds one exact depth per entry
+// in a nibble (8 entries per u32, 0xF = unvisited); the solving formats are
+// encoded from it afterwards.  The reference keeps depth mod 3 in 2 bits and
+// updates its block table with atomics during the search (kernels.cu:389-611);
+// exact nibbles cost 2x the bytes (15 GB for d4t, small next to 288 GB of
+// HBM) and turn both encoders into streaming passes.
+
+constexpr uint32_t BFS_WG = 256;
+
+__device__ __forceinline__ uint32_t nib_get(const uint32_t* tbl, uint64_t idx) {
+    return (tbl[idx >> 3] >> ((uint32_t)(idx & 7u) * 4u)) & 15u;
+}
+
+// Claim the canonical index of nb and its stabiliser images at `depth`
+// (kernels.cu:507-528 claims the same set).  Returns the number newly set.
+template <int K>
+__device__ uint32_t claim_images(const CubeLUT& L, uint32_t* tbl, const uint32_t* __restrict__ classidx,
+                                 const uint16_t* __restrict__ class_stab, const Frame& nb, uint32_t depth) {
+    uint32_t packed = classidx[corner_raw(L, nb)];
+    uint32_t cls = packed >> 4, sym = packed & 15u;
+    uint32_t stab = class_stab[cls];
+    uint64_t base = (uint64_t)cls * KindT<K>::N_EDGE;
+    uint32_t claimed = 0;
+    while (stab) {
+        uint32_t t = ctz(stab);
+        stab &= stab - 1;
+        uint64_t idx = base + edge_coord<K>(L, nb, L.s_mul[t][sym]);
+        uint32_t* wp = tbl + (idx >> 3);
+        uint32_t sh = (uint32_t)(idx & 7u) * 4u;
+        if (((*wp >> sh) & 15u) != 15u) continue;
+        uint32_t old = atomicAnd(wp, ~((15u ^ depth) << sh));
+        claimed += ((old >> sh) & 15u) == 15u;
+    }
+    return claimed;
+}
+
+template <int K>
+__global__ void bfs_seed_kernel(const CubeLUT* __restrict__ gL, uint32_t* tbl, const uint32_t* __restrict__ classidx,
+                                const uint16_t* __restrict__ class_stab, unsigned long long* counter) {
+    __shared__ CubeLUT L;
+    load_lut(&L, gL);
+    if (threadIdx.x) return;
+    atomicAdd(counter, (unsigned long long)claim_images<K>(L, tbl, classidx, class_stab, solved_frame(), 0u));
+}
+
+// One BFS layer.  FWD: expand entries at `depth` and claim their neighbours
+// at depth + 1 (top-down).  !FWD: an unvisited entry with a neighbour at
+// `depth` is at depth + 1 (bottom-up).  Each wave loads 64 consecutive words,
+// compacts the matching entries into an LDS list and shares them across its
+// 64 lanes, so a word with 8 hits costs the same as 8 words with one.
+template <int K, bool FWD>
+__global__ void __launch_bounds__(BFS_WG) bfs_layer_kernel(const CubeLUT* __restrict__ gL, uint32_t* tbl,
+                                                          const uint32_t* __restrict__ classidx,
+                                                          const uint32_t* __restrict__ class_rep,
+                                                          const uint16_t* __restrict__ class_stab,
+                                                          unsigned long long n_words, unsigned long long n_entries,
+                                                          uint32_t depth, unsigned long long* counter) {
+    __shared__ CubeLUT L;
+    __shared__ uint16_t lst[BFS_WG / 64][512];
+    __shared__ uint32_t cnt[BFS_WG / 64];
+    __shared__ uint32_t clr[BFS_WG / 64][64];
+    load_lut(&L, gL);
+    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t next = depth + 1u;
+    const uint32_t pat = depth * 0x11111111u;
+    unsigned long long found = 0;
+    const unsigned long long n_chunks = (n_words + 63) / 64;
+    const unsigned long long stride = (unsigned long long)gridDim.x * (BFS_WG / 64);
+    for (unsigned long long chunk = (unsigned long long)blockIdx.x * (BFS_WG / 64) + wave; chunk < n_chunks;
+         chunk += stride) {
+        const unsigned long long w = chunk * 64 + lane;
+        uint32_t word = 0, hits = 0;
+        if (w < n_words) {
+            word = tbl[w];
+            if (FWD) {
+                uint32_t x = word ^ pat;
+                hits = ~(x | (x >> 1) | (x >> 2) | (x >> 3)) & 0x11111111u;
+            } else {
+                hits = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;
+            }
+            unsigned long long first = w * 8u;
+            if (first + 8u > n_entries) {
+                uint32_t valid = (uint32_t)(n_entries - first);  // 1..7
+                hits &= (1u << (4u * valid)) - 1u;
+            }
+        }
+        if (lane == 0) cnt[wave] = 0;
+        clr[wave][lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t off = atomicAdd(&cnt[wave], popc(hits));
+        for (uint32_t h = hits; h; h &= h - 1) lst[wave][off++] = (uint16_t)((lane << 3) | (ctz(h) >> 2));
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t total = cnt[wave];
+        for (uint32_t i = lane; i < total; i += 64) {
+            uint32_t e = lst[wave][i];
+            uint64_t idx = (chunk * 64 + (e >> 3)) * 8u + (e & 7u);
+            Frame s = unrank<K>(L, class_rep, idx);
+            if (FWD) {
+#pragma unroll 1
+                for (int m = 0; m < 18; ++m) {
+                    Frame nb = s;
+                    apply_move(L, nb, m);
+                    found += claim_images<K>(L, tbl, classidx, class_stab, nb, next);
+                }
+            } else {
+#pragma unroll 1
+                for (int m = 0; m < 18; ++m) {
+                    Frame nb = s;
+                    apply_move(L, nb, m);
+                    if (nib_get(tbl, table_index<K>(L, classidx, nb)) == depth) {
+                        atomicOr(&clr[wave][e >> 3], (15u ^ next) << ((e & 7u) * 4u));
+                        ++found;
+                        break;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (!FWD && w < n_words) {
+            uint32_t c = clr[wave][lane];
+            if (c) tbl[w] = word & ~c;  // this lane owns word w during a bottom-up layer
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) found += __shfl_xor(found, o);
+    if (lane == 0 && found) atomicAdd(counter, found);
+}
+
+// Histogram of the nibble values over the valid entries (the reference's
+// count_values self-check, kernels.cu:614-624).
+__global__ void __launch_bounds__(256) histogram_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_words,
+                                                        unsigned long long n_entries,
+                                                        unsigned long long* __restrict__ counts) {
+    uint32_t local[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) local[v] = 0;
+    for (unsigned long long w = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; w < n_words;
+         w += (unsigned long long)gridDim.x * blockDim.x) {
+        uint32_t word = tbl[w];
+        uint32_t valid = 8;
+        if (w * 8u + 8u > n_entries) valid = (uint32_t)(n_entries - w * 8u);
+        for (uint32_t j = 0; j < valid; ++j) {
+            uint32_t v = (word >> (4 * j)) & 15u;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) local[k] += v == (uint32_t)k;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        unsigned long long s = local[v];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane_id() == 0 && s) atomicAdd(&counts[v], s);
+    }
+}
+
+// Block format (kernels.cu:396-406, 485-503 define it): per 124 entries, u8
+// minimum then 2-bit offsets min(v - min, 3) at bit 8 + 2j.
+__global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_entries,
+                                                           unsigned long long b0, unsigned long long nb,
+                                                           uint32_t* __restrict__ out, unsigned int* __restrict__ bad) {
+    unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    unsigned long long first = (b0 + i) * 124u;
+    uint32_t n = (uint32_t)min(124ull, n_entries - first);
+    uint32_t m = 15;
+    for (uint32_t j = 0; j < n; ++j) m = min(m, nib_get(tbl, first + j));
+    uint32_t w[8] = {m, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t unset = m == 15u;
+    for (uint32_t j = 0; j < n; ++j) {
+        uint32_t v = nib_get(tbl, first + j);
+        unset |= v == 15u;
+        uint32_t pos = 8u + 2u * j;
+        w[pos >> 5] |= min(v - m, 3u) << (pos & 31u);
+    }
+    if (unset) atomicOr(bad, 1u);
+    uint4* o = reinterpret_cast<uint4*>(out + i * 8u);
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// The value the base-3 encoder sees for entry x (kernels.cu:1825-1835): the
+// block-table bound, and when its 2-bit offset saturates, the first value at
+// or above min + 3 with the entry's depth mod 3.  That equals the depth only
+// up to min + 5, so it is recomputed here from the exact depth and the
+// entry's 124-block minimum to give the same bytes.
+__device__ __forceinline__ uint32_t b3_source_value(const uint32_t* __restrict__ tbl,
+                                                    const uint32_t* __restrict__ blk, unsigned long long x) {
+    uint32_t d = nib_get(tbl, x);
+    uint32_t mn = blk[(x / 124u) * 8u] & 0xFFu;
+    return d - mn < 3u ? d : mn + 3u + (d - mn - 3u) % 3u;
+}
+
+// Base-3 format (kernels.cu:1839-1869 define it): per 155 entries, header
+// m | c << 6 with the cut c in 0..3 maximising the sum of encoded bounds
+// (lowest c on ties), then 31 bytes of five digits, little-endian base 3.
+__global__ void __launch_bounds__(256) encode_b3_kernel(const uint32_t* __restrict__ tbl,
+                                                        const uint32_t* __restrict__ blk, unsigned long long n_entries,
+                                                        unsigned long long b0, unsigned long long nb,
+                                                        uint32_t* __restrict__ out) {
+    unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    unsigned long long first = (b0 + i) * 155u;
+    uint32_t n = (uint32_t)min(155ull, n_entries - first);
+    uint32_t m = 255;
+    for (uint32_t j = 0; j < n; ++j) m = min(m, b3_source_value(tbl, blk, first + j));
+    uint32_t score[4] = {0, 0, 0, 0};
+    for (uint32_t j = 0; j < n; ++j) {
+        uint32_t v = b3_source_value(tbl, blk, first + j);
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) score[c] += v <= m + c ? m : (v == m + c + 1 ? m + c + 1 : m + c + 2);
+    }
+    uint32_t best = 0, best_score = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c)
+        if (score[c] > best_score) best_score = score[c], best = c;
+    uint32_t w[8] = {m | (best << 6), 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t q = 0; q < 31; ++q) {
+        uint32_t x = 0;
+        for (int k = 4; k >= 0; --k) {
+            uint32_t j = q * 5u + (uint32_t)k, d = 0;
+            if (j < n) {
+                uint32_t v = b3_source_value(tbl, blk, first + j);
+                d = v <= m + best ? 0u : (v == m + best + 1 ? 1u : 2u);
+            }
+            x = x * 3u + d;
+        }
+        uint32_t byte = 1 + q;
+        w[byte >> 2] |= x << ((byte & 3u) * 8u);
+    }
+    uint4* o = reinterpret_cast<uint4*>(out + i * 8u);
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// ---------------------------------------------------------------------------
+// IDA* search: the reference's step machine (kernels.cu:1447-1801) on wave64.
+// Each outer iteration every lane either performs exactly one table probe
+// (frame k of its current child m) or idles; control work (push, pick the
+// next canonical child, pop, fetch a work item) runs between probes.  The
+// per-lane DFS stack (parent bounds, moves) lives in LDS, lane-major.
+//
+// h and child values are packed 6 x 5 bits in one register.
+
+__device__ __forceinline__ uint32_t hget(uint32_t hp, int j) { return (hp >> (5 * j)) & 31u; }
+__device__ __forceinline__ uint32_t hset(uint32_t hp, int j, uint32_t v) {
+    return (hp & ~(31u << (5 * j))) | (v << (5 * j));
+}
+// Axes whose first move is excluded: an inverse frame at the budget
+// (kernels.cu:750-763).
+__device__ __forceinline__ uint32_t excl_axes(uint32_t hp, uint32_t budget) {
+    return (uint32_t)(hget(hp, 3) == budget) | ((uint32_t)(hget(hp, 4) == budget) << 1) |
+           ((uint32_t)(hget(hp, 5) == budget) << 2);
+}
+__device__ __forceinline__ uint32_t axis_mask(uint32_t axes) {
+    return ((axes & 1u) ? 0x00E07u : 0u) | ((axes & 2u) ? 0x07038u : 0u) | ((axes & 4u) ? 0x381C0u : 0u);
+}
+// All three forward (or inverse) frames at the budget rule out every last
+// (first) move (kernels.cu:765-774).
+__device__ __forceinline__ bool all_tight(uint32_t cp, uint32_t budget) {
+    if (budget == 0) return false;
+    uint32_t b3 = budget | (budget << 5) | (budget << 10);
+    return (cp & 0x7FFFu) == b3 || ((cp >> 15) & 0x7FFFu) == b3;
+}
+// Frame probe order: frames at the budget edge, then one below, then the rest
+// (kernels.cu:1398-1410).
+__device__ __forceinline__ uint32_t frame_order(uint32_t hp, uint32_t edge) {
+    uint32_t at = 0, below = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        uint32_t h = hget(hp, j);
+        at |= (uint32_t)(h >= edge) << j;
+        below |= (uint32_t)(h + 1u == edge) << j;
+    }
+    return at | (below << 6) | ((~(at | below) & 0x3Fu) << 12);
+}
+__device__ __forceinline__ int next_frame(uint32_t& rem) {
+    if (!rem) return 6;
+    uint32_t b = ctz(rem);
+    rem &= rem - 1;
+    return (int)(b % 6u);
+}
+// The six frames of a lane as named registers (an array indexed in loops can
+// be left in scratch memory by the compiler).
+struct Frames6 {
+    Frame f0, f1, f2, f3, f4, f5;
+    template <int J>
+    __device__ __forceinline__ Frame& at() {
+        if constexpr (J == 0) return f0;
+        else if constexpr (J == 1) return f1;
+        else if constexpr (J == 2) return f2;
+        else if constexpr (J == 3) return f3;
+        else if constexpr (J == 4) return f4;
+        else return f5;
+    }
+};
+template <int I, int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+    if constexpr (I < N) {
+        fn(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(fn);
+    }
+}
+__device__ __forceinline__ uint32_t sel6(int k, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t e,
+                                         uint32_t g) {
+    uint32_t t = a;
+    t = k == 1 ? b : t;
+    t = k == 2 ? c : t;
+    t = k == 3 ? d : t;
+    t = k == 4 ? e : t;
+    return k == 5 ? g : t;
+}
+// Frame k, selected word by word (whole-struct selects become memory copies).
+__device__ __forceinline__ Frame pick_frame(const Frames6& f, int k) {
+    Frame t;
+    t.c0 = sel6(k, f.f0.c0, f.f1.c0, f.f2.c0, f.f3.c0, f.f4.c0, f.f5.c0);
+    t.c1 = sel6(k, f.f0.c1, f.f1.c1, f.f2.c1, f.f3.c1, f.f4.c1, f.f5.c1);
+    t.e0 = sel6(k, f.f0.e0, f.f1.e0, f.f2.e0, f.f3.e0, f.f4.e0, f.f5.e0);
+    t.e1 = sel6(k, f.f0.e1, f.f1.e1, f.f2.e1, f.f3.e1, f.f4.e1, f.f5.e1);
+    t.e2 = sel6(k, f.f0.e2, f.f1.e2, f.f2.e2, f.f3.e2, f.f4.e2, f.f5.e2);
+    return t;
+}
+__device__ __forceinline__ Frame load_frame(const Frame* p) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+    return Frame{w[0], w[1], w[2], w[3], w[4]};
+}
+__device__ __forceinline__ void frame_do(const CubeLUT& L, Frame& f, int j, int m) {
+    if (j < 3) apply_move(L, f, L.frame_move[j][m]);
+    else apply_left(L, f, inverse_move(L.frame_move[j - 3][m]));
+}
+__device__ __forceinline__ void frame_undo(const CubeLUT& L, Frame& f, int j, int m) {
+    if (j < 3) apply_move(L, f, inverse_move(L.frame_move[j][m]));
+    else apply_left(L, f, L.frame_move[j - 3][m]);
+}
+
+// Ticket of the persistent form: one 64-bit word per cube packs the round
+// (bound, prefix length), the items handed out and the items in flight, so a
+// single atomic both takes items and keeps the round open.
+constexpr unsigned long long T_NEXT_MASK = (1ull << 28) - 1;
+constexpr unsigned long long T_INF_ONE = 1ull << 28;
+constexpr unsigned long long T_INF_MASK = ((1ull << 21) - 1) << 28;
+constexpr int T_BOUND_SHIFT = 49;
+constexpr int T_PLEN_SHIFT = 54;
+constexpr unsigned long long T_CLOSING = 1ull << 57;
+constexpr unsigned long long T_RETIRED = 1ull << 58;  // serving: the cube's slot is free again
+
+__device__ __forceinline__ uint32_t t_next(unsigned long long t) { return (uint32_t)(t & T_NEXT_MASK); }
+__device__ __forceinline__ uint32_t t_inflight(unsigned long long t) { return (uint32_t)((t & T_INF_MASK) >> 28); }
+__device__ __forceinline__ uint32_t t_bound(unsigned long long t) { return (uint32_t)(t >> T_BOUND_SHIFT) & 31u; }
+__device__ __forceinline__ uint32_t t_plen(unsigned long long t) { return (uint32_t)(t >> T_PLEN_SHIFT) & 7u; }
+
+__device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long tk) {
+    SearchJob* J = &A.jobs[c];
+    fence_acquire();
+    uint32_t bound = t_bound(tk);
+    unsigned long long nodes = ld_relaxed(&J->nodes);
+    uint32_t r = J->n_rounds;
+    if (r < (uint32_t)MAX_BOUND) {
+        J->round_bound[r] = bound;
+        J->round_nodes[r] = nodes;
+        J->n_rounds = r + 1;
+    }
+    if (ld_relaxed(&J->found)) return;  // the finder already made the cube final
+    if ((int)bound + 1 > A.max_bound) {
+        __hip_atomic_store(&J->status, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        J->t_final = wall_clock64();
+        fence_release();
+        atomicAdd(&A.q->n_final, 1u);
+        return;
+    }
+    uint32_t nb = bound + 1;
+    uint32_t p = (uint32_t)A.prefix_len + (nodes >= A.deeper0) + (nodes >= A.deeper1);
+    uint32_t plen = min(min(nb, p), (uint32_t)MAX_PLEN);
+    __hip_atomic_store(&J->nodes, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    J->bound = nb;
+    fence_release();
+    unsigned long long cur = ld_relaxed(&J->ticket);
+    for (;;) {  // keep the in-flight count of late fetchers, reset the item cursor
+        unsigned long long nt = (cur & T_INF_MASK) | ((unsigned long long)nb << T_BOUND_SHIFT) |
+                                ((unsigned long long)plen << T_PLEN_SHIFT);
+        unsigned long long prev = atomicCAS(&J->ticket, cur, nt);
+        if (prev == cur) break;
+        cur = prev;
+    }
+    unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
+    __hip_atomic_store(&A.ring[slot % A.ring_size], (slot << 32) | c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool round_drained(const SearchArgs& A, unsigned long long t) {
+    return t_inflight(t) == 0 && !(t & (T_CLOSING | T_RETIRED)) && t_next(t) >= A.pre_n[t_plen(t)];
+}
+
+template <typename T>
+__device__ __forceinline__ T sys_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <typename T>
+__device__ __forceinline__ void sys_store(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Serving: post the result of a final cube whose last in-flight item has
+// drained; its slot can then take a new cube.
+__device__ void post_done(const SearchArgs& A, uint32_t c) {
+    SearchJob* J = &A.jobs[c];
+    fence_acquire();
+    unsigned long long idx = atomicAdd(&A.q->done_next, 1ull);
+    ServeDone* d = &A.done[idx % A.done_size];
+    uint32_t found = ld_relaxed(&J->found);
+    int32_t length = found ? (int32_t)J->length : -1;
+    unsigned long long nodes = 0;
+    uint32_t nr = J->n_rounds;
+    for (uint32_t r = 0; r < nr && r < (uint32_t)MAX_BOUND; ++r) nodes += J->round_nodes[r];
+    if (found && (nr == 0 || J->round_bound[nr - 1] != J->length)) nodes += ld_relaxed(&J->nodes);
+    sys_store(&d->tag, J->tag);
+    sys_store(&d->slot, c);
+    sys_store(&d->length, length);
+    const uint32_t* sol = reinterpret_cast<const uint32_t*>(J->solution);
+    uint32_t* dm = reinterpret_cast<uint32_t*>(d->moves);
+    for (int w = 0; w < MAX_BOUND / 4; ++w) sys_store(&dm[w], sol[w]);
+    sys_store(&d->nodes, nodes);
+    sys_store(&d->t_admit, J->t_admit);
+    sys_store(&d->t_final, J->t_final);
+    __hip_atomic_store(&d->seq, idx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ void try_retire(const SearchArgs& A, uint32_t c) {
+    SearchJob* J = &A.jobs[c];
+    if (!ld_relaxed(&J->found) && ld_relaxed(&J->status) != 2u) return;
+    unsigned long long t = ld_relaxed(&J->ticket);
+    while (t_inflight(t) == 0 && !(t & T_RETIRED)) {
+        unsigned long long prev = atomicCAS(&J->ticket, t, t | T_RETIRED);
+        if (prev == t) {
+            post_done(A, c);
+            return;
+        }
+        t = prev;
+    }
+}
+
+// Give back n in-flight items of cube c; the release that drains a fully
+// handed-out round closes it (exactly one CAS wins).
+__device__ void release_items(const SearchArgs& A, uint32_t c, uint32_t n) {
+    SearchJob* J = &A.jobs[c];
+    unsigned long long now = atomicAdd(&J->ticket, 0ull - (unsigned long long)n * T_INF_ONE) -
+                             (unsigned long long)n * T_INF_ONE;
+    while (round_drained(A, now)) {
+        unsigned long long prev = atomicCAS(&J->ticket, now, now | T_CLOSING);
+        if (prev == now) {
+            close_round(A, c, now);
+            break;
+        }
+        now = prev;
+    }
+    if (A.serving) try_retire(A, c);
+}
+
+__device__ __forceinline__ void report_solution(const SearchArgs& A, SearchJob* J, bool queue, int g,
+                                                const uint8_t* pth, uint32_t stride) {
+    if (atomicCAS(&J->found, 0u, 1u) != 0u) return;
+    J->length = (uint32_t)g;
+    for (int l = 0; l < g; ++l) J->solution[l] = pth[l * stride];
+    if (queue) {
+        J->status = 1;
+        J->t_final = wall_clock64();
+        fence_release();
+        atomicAdd(&A.q->n_final, 1u);
+    }
+}
+
+template <int K, int F>
+__device__ __forceinline__ uint32_t frame_lb_(const CubeLUT& L, const uint32_t* __restrict__ classidx,
+                                              const uint8_t* __restrict__ table, const Frame& f) {
+    return table_lb<F>(table, table_index<K>(L, classidx, f));
+}
+
+// Serving: take one mailbox record into its slot: root bounds of the six
+// frames, a fresh round ticket (keeping the in-flight count of late
+// fetchers), and a ring entry.  Returns false when the mailbox is empty.
+template <int K, int F>
+__device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
+    unsigned long long h = ld_relaxed(&A.q->mb_head);
+    unsigned long long tail = __hip_atomic_load(A.mb_tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (h >= tail) return false;
+    if (atomicCAS(&A.q->mb_head, h, h + 1) != h) return true;  // another wave took it
+    const ServeRec* r = &A.mb[h % A.mb_size];
+    uint32_t c = sys_load(&r->slot);
+    SearchJob* J = &A.jobs[c];
+    uint32_t bound = 0;
+    for (int j = 0; j < 6; ++j) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(&r->frame[j]);
+        Frame f{sys_load(&w[0]), sys_load(&w[1]), sys_load(&w[2]), sys_load(&w[3]), sys_load(&w[4])};
+        J->frame[j] = f;
+        uint32_t v = frame_lb_<K, F>(L, A.classidx, A.table, f);
+        J->h[j] = v;
+        bound = max(bound, v);
+    }
+    J->tag = sys_load(&r->tag);
+    J->bound = bound;
+    J->length = 0;
+    J->n_rounds = 0;
+    J->t_admit = wall_clock64();
+    __hip_atomic_store(&J->nodes, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&J->status, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&J->found, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t plen = min(min(bound, (uint32_t)A.prefix_len), (uint32_t)MAX_PLEN);
+    fence_release();
+    unsigned long long cur = ld_relaxed(&J->ticket);
+    for (;;) {
+        unsigned long long nt = (cur & T_INF_MASK) | ((unsigned long long)bound << T_BOUND_SHIFT) |
+                                ((unsigned long long)plen << T_PLEN_SHIFT);
+        unsigned long long prev = atomicCAS(&J->ticket, cur, nt);
+        if (prev == cur) break;
+        cur = prev;
+    }
+    unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
+    __hip_atomic_store(&A.ring[slot % A.ring_size], (slot << 32) | c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+template <int K, int F>
+__device__ __forceinline__ uint32_t frame_lb(const CubeLUT& L, const uint32_t* __restrict__ classidx,
+                                             const uint8_t* __restrict__ table, const Frame& f) {
+    return table_lb<F>(table, table_index<K>(L, classidx, f));
+}
+
+template <int K, int F, bool QUEUE>
+__global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    CubeLUT& L = *reinterpret_cast<CubeLUT*>(smem);
+    const uint32_t nt = blockDim.x, tid = threadIdx.x;
+    uint32_t* hst = reinterpret_cast<uint32_t*>(smem + sizeof(CubeLUT)) + tid;
+    uint8_t* pth = reinterpret_cast<uint8_t*>(reinterpret_cast<uint32_t*>(smem + sizeof(CubeLUT)) + MAX_REL * nt) + tid;
+    load_lut(&L, A.lut);
+    const uint32_t* __restrict__ classidx = A.classidx;
+    const uint8_t* __restrict__ table = A.table;
+
+    Frames6 f;
+    uint32_t hp = 0, cp = 0;
+    uint32_t c = 0;  // cube of the current item
+    SearchJob* J = nullptr;
+    uint32_t bound = 0, todo = 0, ord = 0, rem = 0, iter = 0, idle = 0;
+    int g = 0, base = 0, last = -1, m = -1, k = 0;
+    unsigned long long nodes = 0, probes = 0, all_nodes = 0, idle_since = 0;
+    bool have = false, done = false;
+
+    for (;;) {
+        bool eval = false;
+        while (!done) {
+            if (m >= 0) {
+                if (k < 6) {  // still evaluating child m
+                    eval = true;
+                    break;
+                }
+                if (k == 6 && !all_tight(cp, bound - (uint32_t)g - 1u)) {  // survived: push
+                    static_for<0, 6>([&](auto J) { frame_do(L, f.template at<J>(), J, m); });
+                    hst[(g - base) * nt] = hp;
+                    pth[g * nt] = (uint8_t)m;
+                    hp = cp;
+                    last = m;
+                    ++g;
+                    if ((uint32_t)g == bound) {
+                        if (is_solved(f.f0)) report_solution(A, J, QUEUE, g, pth, nt);
+                        todo = 0;
+                    } else {
+                        todo = L.amask[m + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
+                        ord = frame_order(hp, bound - (uint32_t)g);
+                    }
+                }
+                m = -1;
+                continue;
+            }
+            if (!have) {
+                // ---- fetch a work item: (cube, bound, prefix length, prefix) ----
+                uint32_t plen, item;
+                if (!QUEUE) {
+                    unsigned long long mask = __ballot(1);
+                    uint32_t rank = __popcll(mask & lanemask_lt());
+                    unsigned long long w0 = 0;
+                    if (rank == 0) w0 = atomicAdd(A.counter, (unsigned long long)__popcll(mask));
+                    unsigned long long w = first_lane_u64(w0) + rank;
+                    if (w >= (unsigned long long)A.n_active * A.n_prefix) {
+                        done = true;
+                        break;
+                    }
+                    c = (uint32_t)A.active[w / A.n_prefix];
+                    item = (uint32_t)(w % A.n_prefix);
+                    J = &A.jobs[c];
+                    if (ld_relaxed(&J->found)) continue;
+                    bound = J->bound;
+                    plen = (uint32_t)A.plen;
+                } else {
+                    if (idle && (--idle)) break;  // back off between empty fetches
+                    unsigned long long mask = __ballot(1);
+                    uint32_t cnt = __popcll(mask), rank = __popcll(mask & lanemask_lt());
+                    uint32_t cc = 0xFFFFFFFFu;
+                    unsigned long long old = 0;
+                    if (rank == 0) {
+                        for (int tries = 0; tries < 32; ++tries) {
+                            unsigned long long h = ld_acquire(&A.q->head), t = ld_acquire(&A.q->tail);
+                            if (h >= t) {
+                                if (A.serving && admit_one<K, F>(A, L)) continue;
+                                break;
+                            }
+                            unsigned long long e = ld_acquire(&A.ring[h % A.ring_size]);
+                            if ((e >> 32) != (h & 0xFFFFFFFFull)) break;  // entry still being published
+                            uint32_t ce = (uint32_t)e;
+                            SearchJob* Je = &A.jobs[ce];
+                            unsigned long long tk = ld_acquire(&Je->ticket);
+                            if ((tk & (T_CLOSING | T_RETIRED)) || t_next(tk) >= A.pre_n[t_plen(tk)] ||
+                                ld_relaxed(&Je->found)) {
+                                atomicCAS(&A.q->head, h, h + 1);
+                                continue;
+                            }
+                            old = atomicAdd(&Je->ticket, (unsigned long long)cnt * (T_INF_ONE + 1ull));
+                            cc = ce;
+                            break;
+                        }
+                    }
+                    cc = first_lane_u32(cc);
+                    old = first_lane_u64(old);
+                    if (cc == 0xFFFFFFFFu) {
+                        if (ld_relaxed(&A.q->n_final) >= A.q->n_jobs || ld_relaxed(&A.q->stop) ||
+                            (A.serving && sys_load(A.host_stop))) {
+                            done = true;
+                        } else {
+                            idle = 64;
+                            // watchdog: a lane that finds no work for idle_limit ends the launch
+                            unsigned long long now = wall_clock64();
+                            if (!idle_since) idle_since = now;
+                            else if (now - idle_since > A.idle_limit) atomicMax(&A.q->stop, 2u);
+                        }
+                        break;
+                    }
+                    idle_since = 0;
+                    item = t_next(old) + rank;
+                    plen = t_plen(old);
+                    bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < A.pre_n[plen];
+                    unsigned long long inval = __ballot(!valid);
+                    if (rank == 0 && inval) release_items(A, cc, __popcll(inval));
+                    if (!valid) continue;
+                    c = cc;
+                    J = &A.jobs[c];
+                    bound = t_bound(old);
+                    fence_acquire();  // the job's frames were published before its ticket
+                    if (ld_relaxed(&J->found)) {
+                        release_items(A, c, 1);
+                        continue;
+                    }
+                }
+                // ---- replay the prefix with all six frames (kernels.cu:1552-1614) ----
+                const uint8_t* pre = (QUEUE ? A.pre_tab[plen] : A.prefixes) + (unsigned long long)item * plen;
+                static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame(&J->frame[j]); });
+                hp = 0;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
+                nodes = 0;
+                g = 0;
+                last = -1;
+                bool ok = true;
+                for (; g < (int)plen && ok; ++g) {
+                    int pm = pre[g];
+                    ++nodes;
+                    uint32_t axis = (uint32_t)((pm / 3) % 3);
+                    if ((excl_axes(hp, bound - (uint32_t)g) >> axis) & 1u) {
+                        ok = false;
+                        break;
+                    }
+                    static_for<0, 6>([&](auto j) {
+                        Frame& fj = f.template at<j>();
+                        frame_do(L, fj, j, pm);
+                        if ((uint32_t)j != 3u + axis) {  // the inherited inverse frame keeps its coset
+                            uint32_t v = frame_lb<K, F>(L, classidx, table, fj);
+                            uint32_t hj = hget(hp, j);
+                            hp = hset(hp, j, (j < 3 && hj) ? max(v, hj - 1u) : v);
+                        }
+                    });
+                    probes += 5;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j)
+                        if ((uint32_t)g + 1u + hget(hp, j) > bound) ok = false;
+                    if (ok && all_tight(hp, bound - (uint32_t)g - 1u)) ok = false;
+                    pth[g * nt] = (uint8_t)pm;
+                    last = pm;
+                }
+                if (!ok) {
+                    all_nodes += nodes;
+                    atomicAdd(&J->nodes, nodes);
+                    if (QUEUE) {
+                        fence_release();
+                        release_items(A, c, 1);
+                    }
+                    continue;
+                }
+                base = g;
+                if ((uint32_t)g == bound) {
+                    if (is_solved(f.f0)) report_solution(A, J, QUEUE, g, pth, nt);
+                    todo = 0;
+                } else {
+                    todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
+                    ord = frame_order(hp, bound - (uint32_t)g);
+                }
+                have = true;
+                continue;
+            }
+            if (todo && !(((++iter & A.poll_mask) == 0) && ld_relaxed(&J->found))) {
+                // ---- pick the next canonical child ----
+                m = (int)ctz(todo);
+                todo &= todo - 1;
+                ++nodes;
+                int inherit = 3 + (m / 3) % 3;
+                cp = hp;  // the inherited inverse frame keeps the parent's value
+                rem = ord & ~(0x41041u << inherit);
+                k = next_frame(rem);
+                eval = true;
+                break;
+            }
+            // ---- node exhausted: pop, or finish the item ----
+            if (g == base || ld_relaxed(&J->found)) {
+                all_nodes += nodes;
+                atomicAdd(&J->nodes, nodes);
+                if (QUEUE) {
+                    fence_release();
+                    release_items(A, c, 1);
+                }
+                have = false;
+                continue;
+            }
+            --g;
+            int pm = pth[g * nt];
+            static_for<0, 6>([&](auto j) { frame_undo(L, f.template at<j>(), j, pm); });
+            hp = hst[(g - base) * nt];
+            last = g > 0 ? (int)pth[(g - 1) * nt] : -1;
+            todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g)) & ~((2u << pm) - 1u);
+            ord = frame_order(hp, bound - (uint32_t)g);
+        }
+        if (__all(done)) break;
+        if (QUEUE && !__any(eval)) {  // the whole wave is waiting for work
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        if (eval) {
+            // ---- one probe: frame k of child m ----
+            uint32_t parent = hget(hp, k);
+            Frame t = pick_frame(f, k);
+            int mk = L.frame_move[k < 3 ? k : k - 3][m];
+            if (k < 3) apply_move(L, t, mk);
+            else apply_left(L, t, inverse_move(mk));
+            uint32_t v = frame_lb<K, F>(L, classidx, table, t);
+            ++probes;
+            if (k < 3 && parent) v = max(v, parent - 1u);
+            cp = hset(cp, k, v);
+            k = ((uint32_t)g + 1u + v > bound) ? 7 : next_frame(rem);
+        }
+    }
+    if (A.stats) {
+        for (int o = 32; o > 0; o >>= 1) {
+            probes += __shfl_xor(probes, o);
+            all_nodes += __shfl_xor(all_nodes, o);
+        }
+        if (lane_id() == 0) {
+            atomicAdd(&A.stats[0], probes);
+            atomicAdd(&A.stats[1], all_nodes);
+        }
+    }
+}
+
+template <int K>
+void* search_fn(int F, bool queue) {
+    if (F == 0) return queue ? (void*)search_kernel<K, 0, true> : (void*)search_kernel<K, 0, false>;
+    return queue ? (void*)search_kernel<K, 1, true> : (void*)search_kernel<K, 1, false>;
+}
+
+#define CCS_KIND_SWITCH(K, CALL)                  \
+    switch (K) {                                  \
+        case 0: CALL(0); break;                   \
+        case 8: CALL(8); break;                   \
+        case 12: CALL(12); break;                 \
+        case 16: CALL(16); break;                 \
+        case 24: CALL(24); break;                 \
+        default: return hipErrorInvalidValue;     \
+    }
+
+}  // namespace
+
+size_t search_smem_bytes(int block) { return sizeof(CubeLUT) + (size_t)block * (MAX_REL * 4 + MAX_BOUND); }
+
+hipError_t search_occupancy(int K, int F, bool queue, int block, int* blocks_per_cu) {
+    void* fn = nullptr;
+#define CALL(KK) fn = search_fn<KK>(F, queue)
+    CCS_KIND_SWITCH(K, CALL)
+#undef CALL
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, block, search_smem_bytes(block));
+}
+
+hipError_t launch_search(int K, int F, bool queue, const SearchArgs& args, int grid, int block, hipStream_t s) {
+    size_t smem = search_smem_bytes(block);
+#define CALL(KK)                                                                                     \
+    if (F == 0) {                                                                                    \
+        if (queue) hipLaunchKernelGGL((search_kernel<KK, 0, true>), dim3(grid), dim3(block), smem, s, args); \
+        else hipLaunchKernelGGL((search_kernel<KK, 0, false>), dim3(grid), dim3(block), smem, s, args);     \
+    } else {                                                                                         \
+        if (queue) hipLaunchKernelGGL((search_kernel<KK, 1, true>), dim3(grid), dim3(block), smem, s, args); \
+        else hipLaunchKernelGGL((search_kernel<KK, 1, false>), dim3(grid), dim3(block), smem, s, args);     \
+    }
+    CCS_KIND_SWITCH(K, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(int K, int F, const CubeLUT* lut, const uint32_t* classidx, const uint8_t* table,
+                        const Frame* in, int n, int move, unsigned long long* idx_out, uint32_t* lb_out,
+                        Frame* state_out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid((n + 255) / 256), block(256);
+#define CALL(KK)                                                                                                 \
+    if (F == 0) hipLaunchKernelGGL((probe_kernel<KK, 0>), grid, block, 0, s, lut, classidx, table, in, n, move, \
+                                   idx_out, lb_out, state_out);                                                  \
+    else hipLaunchKernelGGL((probe_kernel<KK, 1>), grid, block, 0, s, lut, classidx, table, in, n, move, idx_out, \
+                            lb_out, state_out);
+    CCS_KIND_SWITCH(K, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_unrank_check(int K, const CubeLUT* lut, const uint32_t* classidx, const uint32_t* class_rep,
+                               const unsigned long long* idx, int n, unsigned long long* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid((n + 255) / 256), block(256);
+#define CALL(KK) hipLaunchKernelGGL((unrank_kernel<KK>), grid, block, 0, s, lut, classidx, class_rep, idx, n, out);
+    CCS_KIND_SWITCH(K, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_bfs_seed(int K, const CubeLUT* lut, uint32_t* tbl, const uint32_t* classidx,
+                           const uint16_t* class_stab, unsigned long long* counter, hipStream_t s) {
+#define CALL(KK) hipLaunchKernelGGL((bfs_seed_kernel<KK>), dim3(1), dim3(64), 0, s, lut, tbl, classidx, class_stab, counter);
+    CCS_KIND_SWITCH(K, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_bfs_layer(int K, bool forward, int grid, const CubeLUT* lut, uint32_t* tbl, const uint32_t* classidx,
+                            const uint32_t* class_rep, const uint16_t* class_stab, unsigned long long n_words,
+                            unsigned long long n_entries, uint32_t depth, unsigned long long* counter, hipStream_t s) {
+#define CALL(KK)                                                                                                    \
+    if (forward) hipLaunchKernelGGL((bfs_layer_kernel<KK, true>), dim3(grid), dim3(BFS_WG), 0, s, lut, tbl, classidx, \
+                                    class_rep, class_stab, n_words, n_entries, depth, counter);                     \
+    else hipLaunchKernelGGL((bfs_layer_kernel<KK, false>), dim3(grid), dim3(BFS_WG), 0, s, lut, tbl, classidx,        \
+                            class_rep, class_stab, n_words, n_entries, depth, counter);
+    CCS_KIND_SWITCH(K, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_histogram(int grid, const uint32_t* tbl, unsigned long long n_words, unsigned long long n_entries,
+                            unsigned long long* counts, hipStream_t s) {
+    hipLaunchKernelGGL(histogram_kernel, dim3(grid), dim3(256), 0, s, tbl, n_words, n_entries, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode(int format, const uint32_t* tbl, const uint32_t* blk, unsigned long long n_entries,
+                         unsigned long long b0, unsigned long long nb, uint32_t* out, unsigned int* bad, hipStream_t s) {
+    if (nb == 0) return hipSuccess;
+    dim3 grid((unsigned)((nb + 255) / 256)), block(256);
+    if (format == 0) hipLaunchKernelGGL(encode_block_kernel, grid, block, 0, s, tbl, n_entries, b0, nb, out, bad);
+    else hipLaunchKernelGGL(encode_b3_kernel, grid, block, 0, s, tbl, blk, n_entries, b0, nb, out);
+    return hipGetLastError();
+}
+
+}  // namespace ccs

@@ -1,0 +1,35 @@
+// Launch entry points of the kernels (kernels.hip); K is the table kind (0 for
+// 304, else the number of E-content classes), F the solving-table format
+// (0 block, 1 base 3).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "search_types.hpp"
+
+#ifndef SEARCH_WG
+#define SEARCH_WG 256
+#endif
+
+namespace ccs {
+
+size_t search_smem_bytes(int block);
+hipError_t search_occupancy(int K, int F, bool queue, int block, int* blocks_per_cu);
+hipError_t launch_search(int K, int F, bool queue, const SearchArgs& args, int grid, int block, hipStream_t s);
+hipError_t launch_probe(int K, int F, const CubeLUT* lut, const uint32_t* classidx, const uint8_t* table,
+                        const Frame* in, int n, int move, unsigned long long* idx_out, uint32_t* lb_out,
+                        Frame* state_out, hipStream_t s);
+hipError_t launch_unrank_check(int K, const CubeLUT* lut, const uint32_t* classidx, const uint32_t* class_rep,
+                               const unsigned long long* idx, int n, unsigned long long* out, hipStream_t s);
+hipError_t launch_bfs_seed(int K, const CubeLUT* lut, uint32_t* tbl, const uint32_t* classidx,
+                           const uint16_t* class_stab, unsigned long long* counter, hipStream_t s);
+hipError_t launch_bfs_layer(int K, bool forward, int grid, const CubeLUT* lut, uint32_t* tbl, const uint32_t* classidx,
+                            const uint32_t* class_rep, const uint16_t* class_stab, unsigned long long n_words,
+                            unsigned long long n_entries, uint32_t depth, unsigned long long* counter, hipStream_t s);
+hipError_t launch_histogram(int grid, const uint32_t* tbl, unsigned long long n_words, unsigned long long n_entries,
+                            unsigned long long* counts, hipStream_t s);
+// format 0 writes the block table; format 1 the base-3 table, which reads the
+// block table (blk) for the reference's saturated-offset values.
+hipError_t launch_encode(int format, const uint32_t* tbl, const uint32_t* blk, unsigned long long n_entries,
+                         unsigned long long b0, unsigned long long nb, uint32_t* out, unsigned int* bad, hipStream_t s);
+
+}  // namespace ccs

@@ -1,0 +1,54 @@
+// Constant lookup tables shared by every kernel, built on the host and copied
+// into LDS by each workgroup.  Layout is private to this library (the
+// reference's `Tables` blob, kernels.cu:44-77, is laid out for CUDA's
+// __byte_perm nibble selectors; these are selectors for CDNA's v_perm_b32).
+//
+// Packed cube ("frame"): corners in two u32 (byte = cp | co << 3), edges in
+// three u32 (byte = ep | eo << 4), as in the reference (kernels.cu:79-82).
+//
+// v_perm_b32 (__builtin_amdgcn_perm(S0, S1, sel)): result byte i is chosen by
+// selector byte i: 0-3 = byte of S1, 4-7 = byte of S0, 8-11 = sign fill of
+// bytes 1/3/5/7, 12 = 0x00, >= 13 = 0xFF.  Every table byte fed through a
+// selector here is < 0x80, so selectors 8-11 yield zero as well.
+#pragma once
+#include <cstdint>
+
+namespace ccs {
+
+constexpr uint32_t SEL_ZERO = 0x0Cu;
+
+struct alignas(16) CubeLUT {
+    // right multiplication by HTM move m (state then move)
+    uint32_t mc_sel[18][2];   // corner word w: source corner index per byte (perm over c1:c0)
+    uint32_t mc_add[18][2];   // co << 3 added per byte
+    uint32_t me_lo[18][3];    // edge word w: source index < 8 (perm over e1:e0), else SEL_ZERO
+    uint32_t me_hi[18][3];    // source index - 8 for sources >= 8 (perm over e2:e2), else SEL_ZERO
+    uint32_t me_flip[18][3];  // eo << 4 xored per byte
+    // left multiplication (move then state): byte of the move for each cubie
+    uint32_t lc[18][2];       // corners: move.cp[c] | move.co[c] << 3 for c = 0..7
+    uint32_t le[18][3];       // edges:   move.ep[c] | move.eo[c] << 4 for c = 0..11
+    // edge conjugation by UD symmetry s (M g M^-1): position part, then cubie relabel
+    uint32_t cs_lo[16][3];
+    uint32_t cs_hi[16][3];
+    uint32_t cs_flip[16][3];  // flip of the source position, << 4 per byte
+    uint32_t cs_rel[16][3];   // new byte for each cubie c: perm(c) | flip(c) << 4
+    uint32_t csel[16];        // selector compacting the bytes picked by a 4-bit mask to the low end
+    uint16_t slice_lo[64];    // 12C4 lexicographic rank = slice_lo[low 6 bits] + slice_hi[|low|][high 6 bits]
+    uint8_t slice_hi[5][64];
+    uint8_t ccomb_rank[256];  // 8C4 lexicographic rank
+    uint8_t eorder_rank[256]; // 4 two-bit labels (first in low bits) -> permutation rank
+    uint8_t gen_class[384];   // group kinds: E content (order rank * 16 + flips) -> class
+    uint8_t s_mul[16][16];    // index of M_t M_s among the UD symmetries
+    uint8_t ccomb_unrank[72];
+    uint16_t slice_unrank[496];
+    uint8_t eorder_unrank[24];
+    uint16_t gen_rep[2][48];  // class -> E content, by flip parity of the non-E edges
+    uint8_t frame_move[3][18];// move m seen in axis frame k
+    uint8_t pad0[2];
+    uint32_t amask[19];       // canonical successors of the last move (index last + 1), 18-bit mask
+    uint32_t pad1;
+};
+
+static_assert(sizeof(CubeLUT) % 16 == 0, "LUT copied in 16-byte units");
+
+}  // namespace ccs

@@ -1,0 +1,103 @@
+// Structures shared by the host driver and the kernels.
+#pragma once
+#include <cstdint>
+
+#include "device_cube.hpp"
+
+namespace ccs {
+
+constexpr int MAX_BOUND = 24;   // longest searched sequence (the reference's MAX_DEPTH, kernels.cu:700)
+constexpr int MAX_REL = 20;     // stacked DFS levels below a prefix of length >= 4
+constexpr int MAX_PLEN = 7;     // longest work-item prefix
+
+// One cube being searched, resident in HBM (the role of the reference's
+// CubeJob, kernels.cu:702-710, plus the per-cube round state of the
+// persistent form).
+struct alignas(64) SearchJob {
+    Frame frame[6];              // 0..2: R^k g R^-k; 3..5: their inverses
+    uint32_t h[6];               // root lower bounds per frame
+    uint32_t bound;              // current bound (plain form: set by the host)
+    uint32_t found;              // 1 once a lane has claimed a solution
+    uint32_t length;
+    uint8_t solution[MAX_BOUND];
+    uint32_t status;             // persistent form: 0 searching, 1 solved, 2 gave up
+    uint32_t n_rounds;
+    unsigned long long nodes;    // nodes of the current round
+    unsigned long long ticket;   // persistent form: round ticket (see kernels.hip)
+    unsigned long long t_final;  // device clock when the cube became final
+    unsigned long long tag;      // serving: the caller's tag
+    unsigned long long t_admit;  // serving: device clock at admission
+    uint32_t round_bound[MAX_BOUND];
+    unsigned long long round_nodes[MAX_BOUND];
+};
+
+// Control block of the persistent form.
+struct QueueCtl {
+    unsigned long long head;
+    unsigned long long tail;
+    unsigned int n_final;
+    unsigned int n_jobs;
+    unsigned int stop;           // 1: stop requested; 2: no work for idle_limit while cubes were unfinished
+    unsigned int pad;
+    unsigned long long mb_head;  // serving: mailbox records admitted
+    unsigned long long done_next;// serving: completion records claimed
+};
+
+// Serving mailbox record (host -> device, in pinned host memory); seq is
+// written last, as record index + 1.
+struct ServeRec {
+    Frame frame[6];
+    unsigned long long tag;
+    uint32_t slot;
+    uint32_t pad;
+    unsigned long long seq;
+};
+
+// Serving completion record (device -> host, pinned host memory).
+struct ServeDone {
+    unsigned long long tag;
+    uint32_t slot;
+    int32_t length;
+    uint8_t moves[MAX_BOUND];
+    unsigned long long nodes;
+    unsigned long long t_admit;
+    unsigned long long t_final;
+    unsigned long long seq;
+};
+
+struct SearchArgs {
+    const CubeLUT* lut;
+    const uint32_t* classidx;
+    const uint8_t* table;
+    SearchJob* jobs;
+    // plain form: one round of the listed cubes
+    const int* active;
+    int n_active;
+    int plen;
+    const uint8_t* prefixes;
+    unsigned long long n_prefix;
+    unsigned long long* counter;
+    // persistent form
+    QueueCtl* q;
+    unsigned long long* ring;
+    unsigned long long ring_size;
+    const uint8_t* pre_tab[MAX_PLEN + 1];
+    unsigned int pre_n[MAX_PLEN + 1];
+    int prefix_len;
+    int max_bound;
+    unsigned long long deeper0, deeper1;
+    unsigned int poll_mask;      // found flag polled every poll_mask + 1 picks
+    unsigned long long idle_limit;  // persistent form: device-clock ticks a lane may wait for work
+    // serving
+    int serving;
+    const ServeRec* mb;
+    const unsigned long long* mb_tail;   // pinned host: records published
+    unsigned long long mb_size;
+    ServeDone* done;                     // pinned host ring
+    unsigned long long done_size;
+    const unsigned int* host_stop;       // pinned host: 1 = stop
+    // statistics: [0] probes, [1] nodes
+    unsigned long long* stats;
+};
+
+}  // namespace ccs

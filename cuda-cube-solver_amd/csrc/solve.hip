@@ -1,0 +1,369 @@
+// ccs_solve: IDA* over a resident table.
+//
+// mode 0 (plain) keeps the reference's structure (solver.py:254-320): the host
+// runs the rounds, launching one persistent grid per prefix-length group per
+// round and synchronising in between.  mode 1 (persistent batched) hands
+// every cube's rounds to the device: one launch, each cube advances its own
+// bound as soon as its previous round drains, cubes of very different
+// difficulty share the GPU without host round trips.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "kernels.hpp"
+
+using namespace ccs;
+
+namespace ccs {
+
+namespace {
+
+__global__ void read_clock_kernel(unsigned long long* out) { *out = wall_clock64(); }
+
+struct DeviceBuffers {
+    std::vector<void*> ptrs;
+    template <typename T>
+    hipError_t alloc(T** p, size_t bytes) {
+        hipError_t e = hipMalloc((void**)p, bytes);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+    ~DeviceBuffers() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+void fill_opts(const ccs_solve_opts_t* in, ccs_solve_opts_t& o) {
+    memset(&o, 0, sizeof o);
+    o.mode = 0;
+    o.prefix_len = 4;
+    o.deeper_at[0] = 5e7;
+    o.deeper_at[1] = 3e9;
+    if (in) o = *in;
+    if (o.prefix_len == 0) o.prefix_len = 4;
+    if (o.block == 0) o.block = SEARCH_WG;
+    if (o.max_bound == 0) o.max_bound = MAX_BOUND;
+    if (o.poll_interval == 0) o.poll_interval = 1024;
+    if (o.deeper_at[0] == 0 && o.deeper_at[1] == 0) o.deeper_at[0] = 5e7, o.deeper_at[1] = 3e9;
+}
+
+unsigned long long to_u64(double x) { return x >= 1.8e19 ? ~0ull : (unsigned long long)x; }
+
+}  // namespace
+
+// Grid of a persistent search launch.
+int search_grid(ccs_tables* t, bool queue, int block, int blocks_per_cu, int* grid) {
+    int per_cu = blocks_per_cu;
+    if (per_cu <= 0) {
+        HIP_TRY(search_occupancy(t->K, t->format, queue, block, &per_cu));
+        if (per_cu <= 0) return fail(CCS_EDEVICE, "search kernel does not fit on a CU");
+    }
+    *grid = per_cu * device_cus();
+    return CCS_OK;
+}
+
+int root_setup(ccs_tables* t, const uint8_t* cubes, int n, std::vector<SearchJob>& jobs, std::vector<int>& solved0) {
+    jobs.assign(n, SearchJob{});
+    std::vector<Packed> frames(6ull * n);
+    for (int i = 0; i < n; ++i) {
+        const Cubie& c = *reinterpret_cast<const Cubie*>(cubes + 40ull * i);
+        if (!is_legal(c)) return fail(CCS_EINVAL, "cube " + std::to_string(i) + " is not a legal cube");
+        Cubie f[6];
+        axis_frames(c, f);
+        for (int j = 0; j < 6; ++j) frames[6ull * i + j] = pack(f[j]);
+        Cubie s = solved();
+        solved0[i] = memcmp(&c, &s, sizeof c) == 0;
+    }
+    DeviceBuffers buf;
+    Frame* d_frames = nullptr;
+    uint32_t* d_lb = nullptr;
+    HIP_TRY(buf.alloc(&d_frames, sizeof(Frame) * 6ull * n));
+    HIP_TRY(buf.alloc(&d_lb, 4ull * 6 * n));
+    HIP_TRY(hipMemcpyAsync(d_frames, frames.data(), sizeof(Frame) * 6ull * n, hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(launch_probe(t->K, t->format, t->d_lut, t->d_classidx, t->d_table[t->format], d_frames, 6 * n, -1, nullptr,
+                         d_lb, nullptr, t->stream));
+    std::vector<uint32_t> lb(6ull * n);
+    HIP_TRY(hipMemcpyAsync(lb.data(), d_lb, 4ull * 6 * n, hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    for (int i = 0; i < n; ++i) {
+        SearchJob& J = jobs[i];
+        memset(&J, 0, sizeof J);
+        uint32_t b = 0;
+        for (int j = 0; j < 6; ++j) {
+            const Packed& p = frames[6ull * i + j];
+            J.frame[j] = Frame{p.w[0], p.w[1], p.w[2], p.w[3], p.w[4]};
+            J.h[j] = lb[6ull * i + j];
+            b = std::max(b, J.h[j]);
+        }
+        J.bound = b;  // the largest frame bound (solver.py:261)
+    }
+    return CCS_OK;
+}
+
+}  // namespace ccs
+
+extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs_solve_opts_t* opts_in,
+                         ccs_result_t* out, ccs_solve_stats_t* stats) {
+    if (!t || n < 0 || (n && (!cubes || !out))) return fail(CCS_EINVAL, "bad arguments");
+    if (t->format < 0 || !t->d_table[t->format]) return fail(CCS_EINVAL, "no solving table resident");
+    ccs_solve_opts_t o;
+    fill_opts(opts_in, o);
+    if (o.mode != 0 && o.mode != 1) return fail(CCS_EINVAL, "mode must be 0 (plain) or 1 (persistent)");
+    if (o.prefix_len < 4 || o.prefix_len > MAX_PLEN) return fail(CCS_EINVAL, "prefix_len must be 4..7");
+    if (o.max_bound < 1 || o.max_bound > MAX_BOUND) return fail(CCS_EINVAL, "max_bound must be 1..24");
+    if (o.block != SEARCH_WG) return fail(CCS_EINVAL, "block must be " + std::to_string(SEARCH_WG));
+    if (o.poll_interval & (o.poll_interval - 1)) return fail(CCS_EINVAL, "poll_interval must be a power of two");
+    double t_start = now_ms();
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (n == 0) return CCS_OK;
+    for (int i = 0; i < n; ++i) {
+        memset(&out[i], 0, sizeof out[i]);
+        out[i].length = -1;
+    }
+    std::vector<SearchJob> jobs;
+    std::vector<int> solved0(n, 0);
+    int rc = root_setup(t, cubes, n, jobs, solved0);
+    if (rc) return rc;
+    const bool queue = o.mode == 1;
+    int grid = 0;
+    rc = search_grid(t, queue, o.block, o.blocks_per_cu, &grid);
+    if (rc) return rc;
+    const unsigned long long d0 = to_u64(o.deeper_at[0]), d1 = to_u64(o.deeper_at[1]);
+    for (int p = 0; p <= MAX_PLEN; ++p) {
+        if (p > o.prefix_len + 2) break;
+        if ((rc = ensure_prefixes(t, p))) return rc;
+    }
+
+    DeviceBuffers buf;
+    SearchJob* d_jobs = nullptr;
+    unsigned long long* d_stats = nullptr;
+    HIP_TRY(buf.alloc(&d_jobs, sizeof(SearchJob) * (size_t)n));
+    HIP_TRY(buf.alloc(&d_stats, 4 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), t->stream));
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    double kernel_ms = 0;
+    uint32_t launches = 0;
+
+    SearchArgs A;
+    memset(&A, 0, sizeof A);
+    A.lut = t->d_lut;
+    A.classidx = t->d_classidx;
+    A.table = t->d_table[t->format];
+    A.jobs = d_jobs;
+    A.prefix_len = o.prefix_len;
+    A.max_bound = o.max_bound;
+    A.deeper0 = d0;
+    A.deeper1 = d1;
+    A.poll_mask = (unsigned)o.poll_interval - 1u;
+    A.stats = d_stats;
+    for (int p = 0; p <= MAX_PLEN; ++p) {
+        A.pre_tab[p] = t->d_prefix[p];
+        A.pre_n[p] = t->prefix_n[p] ? t->prefix_n[p] : 0xFFFFFFFFu;
+    }
+
+    std::vector<double> done_ms(n, -1.0);
+    if (!queue) {
+        // ---- plain: host-driven rounds (solver.py:270-316) ----
+        int* d_active = nullptr;
+        unsigned long long* d_counter = nullptr;
+        HIP_TRY(buf.alloc(&d_active, sizeof(int) * (size_t)n));
+        HIP_TRY(buf.alloc(&d_counter, sizeof(unsigned long long)));
+        HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(SearchJob) * (size_t)n, hipMemcpyHostToDevice, t->stream));
+        std::vector<unsigned long long> last(n, 0);
+        std::vector<int> finished(n, 0);
+        for (int i = 0; i < n; ++i)
+            if (solved0[i]) finished[i] = 1, done_ms[i] = now_ms() - t_start;
+        for (;;) {
+            std::map<int, std::vector<int>> groups;
+            for (int i = 0; i < n; ++i) {
+                if (finished[i]) continue;
+                if ((int)jobs[i].bound > o.max_bound) {
+                    finished[i] = 1;
+                    continue;
+                }
+                int plen = o.prefix_len + (last[i] >= d0) + (last[i] >= d1);
+                plen = std::min(std::min(plen, (int)jobs[i].bound), MAX_PLEN);
+                groups[plen].push_back(i);
+            }
+            if (groups.empty()) break;
+            std::vector<int> ids;
+            HIP_TRY(hipEventRecord(ev0, t->stream));
+            for (auto& kv : groups) {
+                int plen = kv.first;
+                if ((rc = ensure_prefixes(t, plen))) return rc;
+                size_t off = ids.size();
+                ids.insert(ids.end(), kv.second.begin(), kv.second.end());
+                (void)off;
+            }
+            HIP_TRY(hipMemcpyAsync(d_active, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, t->stream));
+            size_t off = 0;
+            for (auto& kv : groups) {
+                int plen = kv.first;
+                HIP_TRY(hipMemsetAsync(d_counter, 0, sizeof(unsigned long long), t->stream));
+                A.active = d_active + off;
+                A.n_active = (int)kv.second.size();
+                A.plen = plen;
+                A.prefixes = t->d_prefix[plen];
+                A.n_prefix = t->prefix_n[plen];
+                A.counter = d_counter;
+                HIP_TRY(launch_search(t->K, t->format, false, A, grid, o.block, t->stream));
+                ++launches;
+                off += kv.second.size();
+            }
+            HIP_TRY(hipEventRecord(ev1, t->stream));
+            HIP_TRY(hipMemcpyAsync(jobs.data(), d_jobs, sizeof(SearchJob) * (size_t)n, hipMemcpyDeviceToHost, t->stream));
+            HIP_TRY(hipStreamSynchronize(t->stream));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+            kernel_ms += ms;
+            double now = now_ms() - t_start;
+            for (int i : ids) {
+                SearchJob& J = jobs[i];
+                ccs_result_t& r = out[i];
+                if (r.n_rounds < 24) {
+                    r.round_bound[r.n_rounds] = J.bound;
+                    r.round_nodes[r.n_rounds] = J.nodes;
+                    ++r.n_rounds;
+                }
+                r.nodes += J.nodes;
+                last[i] = J.nodes;
+                J.nodes = 0;
+                if (J.found) {
+                    finished[i] = 1;
+                    done_ms[i] = now;
+                } else {
+                    J.bound += 1;
+                }
+            }
+            HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(SearchJob) * (size_t)n, hipMemcpyHostToDevice, t->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(t->stream));
+    } else {
+        // ---- persistent batched: all rounds on the device ----
+        std::vector<int> live;
+        for (int i = 0; i < n; ++i) {
+            if (solved0[i]) {
+                done_ms[i] = 0;
+                continue;
+            }
+            live.push_back(i);
+        }
+        const int nl = (int)live.size();
+        std::vector<SearchJob> lj(nl);
+        for (int k = 0; k < nl; ++k) {
+            lj[k] = jobs[live[k]];
+            uint32_t b = lj[k].bound;
+            uint32_t plen = std::min(std::min(b, (uint32_t)o.prefix_len), (uint32_t)MAX_PLEN);
+            lj[k].ticket = ((unsigned long long)b << 49) | ((unsigned long long)plen << 54);
+        }
+        const unsigned long long ring_size = 2ull * nl + 1024;
+        std::vector<unsigned long long> ring(ring_size, ~0ull);
+        for (int k = 0; k < nl; ++k) ring[k] = ((unsigned long long)k << 32) | (unsigned)k;
+        QueueCtl q{};
+        q.head = 0;
+        q.tail = (unsigned long long)nl;
+        q.n_jobs = (unsigned)nl;
+        QueueCtl* d_q = nullptr;
+        unsigned long long *d_ring = nullptr, *d_clock = nullptr;
+        HIP_TRY(buf.alloc(&d_q, sizeof(QueueCtl)));
+        HIP_TRY(buf.alloc(&d_ring, sizeof(unsigned long long) * ring_size));
+        HIP_TRY(buf.alloc(&d_clock, sizeof(unsigned long long)));
+        if (nl) {
+            HIP_TRY(hipMemcpyAsync(d_jobs, lj.data(), sizeof(SearchJob) * (size_t)nl, hipMemcpyHostToDevice, t->stream));
+            HIP_TRY(hipMemcpyAsync(d_ring, ring.data(), sizeof(unsigned long long) * ring_size, hipMemcpyHostToDevice,
+                                   t->stream));
+            HIP_TRY(hipMemcpyAsync(d_q, &q, sizeof q, hipMemcpyHostToDevice, t->stream));
+            A.q = d_q;
+            A.ring = d_ring;
+            A.ring_size = ring_size;
+            int rate_khz0 = 0, dev0 = 0;
+            HIP_TRY(hipGetDevice(&dev0));
+            HIP_TRY(hipDeviceGetAttribute(&rate_khz0, hipDeviceAttributeWallClockRate, dev0));
+            A.idle_limit = (unsigned long long)std::max(rate_khz0, 1) * 60000ull;  // 60 s
+            hipLaunchKernelGGL(read_clock_kernel, dim3(1), dim3(1), 0, t->stream, d_clock);
+            HIP_TRY(hipEventRecord(ev0, t->stream));
+            HIP_TRY(launch_search(t->K, t->format, true, A, grid, o.block, t->stream));
+            ++launches;
+            HIP_TRY(hipEventRecord(ev1, t->stream));
+            HIP_TRY(hipMemcpyAsync(lj.data(), d_jobs, sizeof(SearchJob) * (size_t)nl, hipMemcpyDeviceToHost, t->stream));
+            unsigned long long clk0 = 0;
+            HIP_TRY(hipMemcpyAsync(&clk0, d_clock, 8, hipMemcpyDeviceToHost, t->stream));
+            HIP_TRY(hipMemcpyAsync(&q, d_q, sizeof q, hipMemcpyDeviceToHost, t->stream));
+            HIP_TRY(hipStreamSynchronize(t->stream));
+            if (q.stop == 2) return fail(CCS_EFAULT, "persistent search stalled: no work for 60 s with cubes unfinished");
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+            kernel_ms += ms;
+            int rate_khz = 0;
+            int dev = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+            double launch_ms = now_ms() - t_start - ms;  // host time before the kernel started
+            for (int k = 0; k < nl; ++k) {
+                SearchJob& J = lj[k];
+                int i = live[k];
+                ccs_result_t& r = out[i];
+                r.n_rounds = std::min(J.n_rounds, 24u);
+                for (uint32_t j = 0; j < r.n_rounds; ++j) {
+                    r.round_bound[j] = J.round_bound[j];
+                    r.round_nodes[j] = J.round_nodes[j];
+                    r.nodes += J.round_nodes[j];
+                }
+                if (J.found && (r.n_rounds == 0 || r.round_bound[r.n_rounds - 1] != J.length) && r.n_rounds < 24) {
+                    // the solving round ends when the solution is claimed
+                    r.round_bound[r.n_rounds] = J.length;
+                    r.round_nodes[r.n_rounds] = J.nodes;
+                    r.nodes += J.nodes;
+                    ++r.n_rounds;
+                }
+                jobs[i].found = J.found;
+                jobs[i].length = J.length;
+                memcpy(jobs[i].solution, J.solution, sizeof J.solution);
+                if (J.status && rate_khz > 0)
+                    done_ms[i] = launch_ms + (double)(J.t_final - clk0) / (double)rate_khz;
+            }
+        }
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+
+    // Verify every solution (solver.py:317-319).
+    int bad = -1;
+    for (int i = 0; i < n; ++i) {
+        ccs_result_t& r = out[i];
+        r.seconds = done_ms[i] / 1e3;
+        if (solved0[i]) {
+            r.length = 0;
+            continue;
+        }
+        if (!jobs[i].found) continue;
+        r.length = (int)jobs[i].length;
+        Cubie c = *reinterpret_cast<const Cubie*>(cubes + 40ull * i);
+        for (int l = 0; l < r.length; ++l) {
+            r.moves[l] = jobs[i].solution[l];
+            c = then(c, move(r.moves[l]));
+        }
+        Cubie s = solved();
+        if (memcmp(&c, &s, sizeof c) != 0 && bad < 0) bad = i;
+    }
+    if (stats) {
+        unsigned long long hs[4];
+        HIP_TRY(hipMemcpy(hs, d_stats, sizeof hs, hipMemcpyDeviceToHost));
+        stats->probes = hs[0];
+        stats->nodes = hs[1];
+        stats->kernel_ms = kernel_ms;
+        stats->launches = launches;
+        stats->wall_ms = now_ms() - t_start;
+    }
+    if (bad >= 0) return fail(CCS_EFAULT, "solution of cube " + std::to_string(bad) + " does not solve it");
+    for (int i = 0; i < n; ++i)
+        if (out[i].length < 0) return fail(CCS_EFAULT, "cube " + std::to_string(i) + " not solved within max_bound");
+    return CCS_OK;
+}

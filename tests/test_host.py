@@ -1,0 +1,149 @@
+"""Host-side logic of the product (no GPU): the Python mirror of the
+reference's interface, the C-ABI library's exports and the LUT arithmetic
+the kernels use, evaluated on the CPU."""
+import ctypes
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cuda_cube_solver_amd import _lib, cubie, facelets, kinds, states
+from cuda_cube_solver_amd.cubie import MOVES_HTM, CubieState, apply_moves
+
+ROOT = Path(__file__).resolve().parents[1]
+SOLVED = "UUUUUUUUURRRRRRRRRFFFFFFFFFDDDDDDDDDLLLLLLLLLBBBBBBBBB"
+
+
+def _state(d):
+    return CubieState(tuple(d["cp"]), tuple(d["co"]), tuple(d["ep"]), tuple(d["eo"]))
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "cubesolver.h").read_text()
+    declared = set(re.findall(r"\b(ccs_[a-z0-9_]+)\s*\(", header))
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    L = _lib.lib()
+    missing = [name for name in declared if not hasattr(L, name)]
+    assert not missing
+    assert L.ccs_version() >= 100
+
+
+def test_moves_match_reference(index_vectors):
+    for m, want in zip(MOVES_HTM, index_vectors["moves"]):
+        assert [list(getattr(m, f)) for f in ("cp", "co", "ep", "eo")] == [want[f] for f in ("cp", "co", "ep", "eo")]
+
+
+@pytest.mark.parametrize("name", ["304", "d4t", "d4", "s4", "d4f", "a4t", "a4"])
+def test_kernel_lut_arithmetic_matches_reference_index(index_vectors, name):
+    """The kernels' v_perm/LDS index arithmetic (evaluated on the host) gives
+    the reference's table index for every kind."""
+    arr = O.cubes(index_vectors["states"])
+    out = np.zeros(len(arr), dtype=np.uint64)
+    _lib.check(_lib.lib().ccs_index_host(name.encode(), arr.ctypes.data, len(arr), out.ctypes.data))
+    assert (out == np.array(index_vectors["kinds"][name]["table_index"], dtype=np.uint64)).all()
+
+
+def test_kernel_lut_arithmetic_matches_oracle_on_seeded_states():
+    sts = states.uniform_states(300, 99) + states.flip_heavy_states(20, 4)
+    arr = O.cubes([(s.cp, s.co, s.ep, s.eo) for s in sts])
+    for name in ("304", "d4t", "d4"):
+        out = np.zeros(len(arr), dtype=np.uint64)
+        _lib.check(_lib.lib().ccs_index_host(name.encode(), arr.ctypes.data, len(arr), out.ctypes.data))
+        assert (out == O.table_index(O.kind(name), arr)).all()
+
+
+def test_kind_info():
+    for name in ("304", "d4t", "d4", "s4", "d4f", "a4t", "a4"):
+        k = _lib.kind_info(name)
+        assert k.n_edge == kinds.n_edge(name) == O.kind(name).n_edge
+        assert k.n_entries == 9930 * k.n_edge
+    with pytest.raises(_lib.CcsError):
+        _lib.kind_info("nope")
+
+
+def test_presets_match_reference(index_vectors):
+    for want in index_vectors["presets"]:
+        p = kinds.PRESET_BY_NAME[want["name"]]
+        assert (p.kind, p.format, p.file, p.entries, p.table_bytes, p.auto) == \
+            (want["kind"], want["format"], want["file"], want["entries"], want["table_bytes"], want["auto"])
+    assert [p.name for p in kinds.PRESETS] == [w["name"] for w in index_vectors["presets"]]
+
+
+def test_preset_choice(tmp_path):
+    names = lambda pair: tuple(p.name if p else None for p in pair)
+    # MI355X-sized budgets: everything builds
+    assert names(kinds.choose(tmp_path, int(200e9))) == ("d4t", None)
+    # a 16 GB budget holds d4t but cannot build it (exact-depth build table, 4 bits/entry)
+    assert names(kinds.choose(tmp_path, int(16e9))) == ("a4t", "d4t")
+    (tmp_path / "prune-ud-d4t-blk.bin").touch()
+    assert names(kinds.choose(tmp_path, int(16e9))) == ("d4t", None)
+    assert names(kinds.choose(tmp_path, int(16e9), to_build=True)) == ("a4t", "d4t")
+    assert names(kinds.choose(tmp_path, int(1e9))) == (None, None)
+    with pytest.raises(ValueError, match="unknown table"):
+        kinds.pick(tmp_path, name="nope")
+
+
+def test_facelets_match_reference(index_vectors):
+    sts = [_state(d) for d in index_vectors["states"][:50]]
+    for s, f in zip(sts, index_vectors["facelets"]):
+        assert facelets.to_facelets(s) == f
+        assert facelets.parse_facelets(f) == s
+    assert facelets.parse_facelets(SOLVED) == CubieState()
+
+
+def _swap(s, *idx):
+    n = len(idx) // 2
+    out = list(s)
+    for t, src in zip(idx[:n], idx[n:]):
+        out[t] = s[src]
+    return "".join(out)
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (lambda s: s[:53], "54 stickers"),
+    (lambda s: s.replace("U", "R", 1), "has 8 stickers"),
+    (lambda s: _swap(s, 8, 9, 20, 9, 20, 8), "twisted"),
+    (lambda s: _swap(s, 5, 10, 10, 5), "flipped"),
+    (lambda s: _swap(s, 5, 7, 10, 19, 7, 5, 19, 10), "swapped"),
+])
+def test_illegal_facelets(mutate, message):
+    with pytest.raises(facelets.FaceletError, match=message):
+        facelets.parse_facelets(mutate(SOLVED))
+
+
+def test_colour_letters():
+    s = apply_moves(CubieState(), [MOVES_HTM[m] for m in cubie.parse_moves("R U2 F' L D B2")])
+    col = facelets.to_facelets(s).translate(str.maketrans("URFDLB", "WRGYOB"))
+    assert facelets.is_facelet_string(col) and facelets.parse_facelets(col) == s
+
+
+def test_seeded_states_are_legal_and_deterministic():
+    a = states.uniform_states(200, 3)
+    assert a == states.uniform_states(200, 3) and a != states.uniform_states(200, 4)
+    assert all(s.is_legal() for s in a)
+    # roughly uniform twist of corner 0 and flip of edge 0
+    assert 40 < sum(s.co[0] == 1 for s in a) < 100
+    f = states.flip_heavy_states(16, 4)
+    assert all(s.is_legal() and s.cp == tuple(range(8)) and s.ep == tuple(range(12)) for s in f)
+    assert f[-1].eo == (1,) * 12
+
+
+def test_cli_reports_bad_lines_in_order(tmp_path):
+    """Input errors print `<n> error <reason>` (no table needed when every line is bad)."""
+    env = {"PYTHONPATH": str(ROOT / "cuda-cube-solver_amd")}
+    out = subprocess.run([sys.executable, "-m", "cuda_cube_solver_amd", "solve", "--tables", str(tmp_path)],
+                         input="R X\n\n# comment\n" + SOLVED.replace("U", "R", 1) + "\n", capture_output=True, text=True,
+                         env=env)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.splitlines() == ["1 error unknown move 'X'", "4 error face U has 8 stickers, expected 9"]
+
+
+def test_cli_unknown_table(tmp_path):
+    env = {"PYTHONPATH": str(ROOT / "cuda-cube-solver_amd")}
+    out = subprocess.run([sys.executable, "-m", "cuda_cube_solver_amd", "solve", "--table", "nope"],
+                         input="R U\n", capture_output=True, text=True, env=env)
+    assert out.returncode != 0 and "unknown table 'nope'" in out.stderr
