@@ -274,6 +274,13 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
     return CCS_OK;
 }
 
+int ccs_tables_use(ccs_tables_t* t, int format) {
+    if (!t || format < 0 || format > 1) return fail(CCS_EINVAL, "bad arguments");
+    if (!t->d_table[format]) return fail(CCS_EINVAL, "that table format is not resident");
+    t->format = format;
+    return CCS_OK;
+}
+
 uint64_t ccs_tables_bytes(ccs_tables_t* t, int format) {
     if (!t || format < 0 || format > 1) return 0;
     return t->table_bytes[format];

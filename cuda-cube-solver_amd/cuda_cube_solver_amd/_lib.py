@@ -86,6 +86,7 @@ SIGNATURES = {
     "ccs_tables_open": (I, [CP, CP, I, P]),
     "ccs_tables_build": (I, [CP, I, LOG_FN, P, P, P, P]),
     "ccs_tables_save": (I, [P, I, CP]),
+    "ccs_tables_use": (I, [P, I]),
     "ccs_tables_bytes": (U64, [P, I]),
     "ccs_tables_read": (I, [P, I, U64, U64, P]),
     "ccs_tables_close": (None, [P]),
@@ -109,9 +110,7 @@ def lib() -> ctypes.CDLL:
             raise LibraryMissing(f"{LIB_PATH} not found; build it with `make -C {PKG_ROOT}`")
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
-            fn = getattr(L, name, None)
-            if fn is None:
-                continue
+            fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
         _lib = L

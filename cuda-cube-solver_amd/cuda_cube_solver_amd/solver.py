@@ -107,6 +107,11 @@ class Tables:
             raise RuntimeError("tables handle is closed")
         return self._h
 
+    def use(self, fmt: str) -> None:
+        """Search with the resident block or b3 table (both are kept after build(..., "both"))."""
+        _lib.check(_lib.lib().ccs_tables_use(self.handle, 1 if fmt == "b3" else 0))
+        self.format = fmt
+
     def nbytes(self, fmt: str | None = None) -> int:
         return int(_lib.lib().ccs_tables_bytes(self.handle, 1 if (fmt or self.format) == "b3" else 0))
 

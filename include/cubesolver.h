@@ -95,6 +95,11 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
  * Replaces save_device / write_b3 and the .tmp rename (table.py:20-25, 66-79, 190-198). */
 int ccs_tables_save(ccs_tables_t* t, int format, const char* path);
 
+/* Choose which resident solving table (CCS_FORMAT_BLOCK or CCS_FORMAT_B3)
+ * probes and searches read; a table built with both formats keeps both.
+ * Replaces picking the table file by format in GpuSolver (solver.py:164-169). */
+int ccs_tables_use(ccs_tables_t* t, int format);
+
 /* Bytes of a resident solving table (0 if that format is not resident). */
 uint64_t ccs_tables_bytes(ccs_tables_t* t, int format);
 
