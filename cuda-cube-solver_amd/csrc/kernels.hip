@@ -32,21 +32,21 @@ __device__ __forceinline__ unsigned long long first_lane_u64(unsigned long long 
     return ((unsigned long long)hi << 32) | lo;
 }
 
-__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
-__device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-
+// Cross-lane state of the persistent form is read and written with relaxed
+// agent-scope atomics: on gfx950 they go past the per-XCD L2 (sc1), so they
+// are coherent across the eight XCDs without the L2 invalidate/writeback
+// that agent-scope acquire/release fences cost.  Ordering between two such
+// accesses of one lane comes from waiting for the first to complete.
 template <typename T>
-__device__ __forceinline__ T ld_acquire(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T ld_relaxed(const T* p) {
+__device__ __forceinline__ T ld_c(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
-__device__ __forceinline__ void st_release(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_c(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Wait until this wave's outstanding memory operations have completed.
+__device__ __forceinline__ void complete() { __builtin_amdgcn_s_waitcnt(0); }
 
 // ---------------------------------------------------------------------------
 // probes
@@ -428,47 +428,62 @@ constexpr int T_BOUND_SHIFT = 49;
 constexpr int T_PLEN_SHIFT = 54;
 constexpr unsigned long long T_CLOSING = 1ull << 57;
 constexpr unsigned long long T_RETIRED = 1ull << 58;  // serving: the cube's slot is free again
+constexpr int T_CHUNK_SHIFT = 59;                     // log2 of the items a lane takes per fetch (0..6)
 
 __device__ __forceinline__ uint32_t t_next(unsigned long long t) { return (uint32_t)(t & T_NEXT_MASK); }
 __device__ __forceinline__ uint32_t t_inflight(unsigned long long t) { return (uint32_t)((t & T_INF_MASK) >> 28); }
 __device__ __forceinline__ uint32_t t_bound(unsigned long long t) { return (uint32_t)(t >> T_BOUND_SHIFT) & 31u; }
 __device__ __forceinline__ uint32_t t_plen(unsigned long long t) { return (uint32_t)(t >> T_PLEN_SHIFT) & 7u; }
+__device__ __forceinline__ uint32_t t_chunk(unsigned long long t) { return (uint32_t)(t >> T_CHUNK_SHIFT) & 7u; }
+__device__ __forceinline__ unsigned long long make_ticket(unsigned long long keep, uint32_t bound, uint32_t plen,
+                                                          uint32_t chunk_log) {
+    return (keep & T_INF_MASK) | ((unsigned long long)bound << T_BOUND_SHIFT) |
+           ((unsigned long long)plen << T_PLEN_SHIFT) | ((unsigned long long)chunk_log << T_CHUNK_SHIFT);
+}
+// Items per fetch for a round: about chunk_nodes nodes per chunk, judged by
+// the previous round's nodes and the usual growth of ~13x per bound.
+__device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned long long last_nodes,
+                                                  uint32_t n_items) {
+    unsigned long long per_item = (last_nodes * 13u) / (n_items ? n_items : 1u) + 1u;
+    uint32_t lg = 0;
+    // at least 4096 chunks per round, so one cube alone still spreads over the GPU
+    while (lg < 6 && (per_item << (lg + 1)) <= A.chunk_nodes && (n_items >> (lg + 1)) >= 4096u) ++lg;
+    return lg;
+}
 
 __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long tk) {
     SearchJob* J = &A.jobs[c];
-    fence_acquire();
     uint32_t bound = t_bound(tk);
-    unsigned long long nodes = ld_relaxed(&J->nodes);
-    uint32_t r = J->n_rounds;
+    unsigned long long nodes = ld_c(&J->nodes);
+    uint32_t r = ld_c(&J->n_rounds);
     if (r < (uint32_t)MAX_BOUND) {
-        J->round_bound[r] = bound;
-        J->round_nodes[r] = nodes;
-        J->n_rounds = r + 1;
+        st_c(&J->round_bound[r], bound);
+        st_c(&J->round_nodes[r], nodes);
+        st_c(&J->n_rounds, r + 1);
     }
-    if (ld_relaxed(&J->found)) return;  // the finder already made the cube final
+    if (ld_c(&J->found)) return;  // the finder already made the cube final
     if ((int)bound + 1 > A.max_bound) {
-        __hip_atomic_store(&J->status, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        J->t_final = wall_clock64();
-        fence_release();
+        st_c(&J->status, 2u);
+        st_c(&J->t_final, (unsigned long long)wall_clock64());
+        complete();
         atomicAdd(&A.q->n_final, 1u);
         return;
     }
     uint32_t nb = bound + 1;
     uint32_t p = (uint32_t)A.prefix_len + (nodes >= A.deeper0) + (nodes >= A.deeper1);
     uint32_t plen = min(min(nb, p), (uint32_t)MAX_PLEN);
-    __hip_atomic_store(&J->nodes, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    J->bound = nb;
-    fence_release();
-    unsigned long long cur = ld_relaxed(&J->ticket);
+    uint32_t clog = chunk_log_for(A, nodes, A.pre_n[plen]);
+    st_c(&J->nodes, 0ull);
+    st_c(&J->bound, nb);
+    complete();
+    unsigned long long cur = ld_c(&J->ticket);
     for (;;) {  // keep the in-flight count of late fetchers, reset the item cursor
-        unsigned long long nt = (cur & T_INF_MASK) | ((unsigned long long)nb << T_BOUND_SHIFT) |
-                                ((unsigned long long)plen << T_PLEN_SHIFT);
-        unsigned long long prev = atomicCAS(&J->ticket, cur, nt);
+        unsigned long long prev = atomicCAS(&J->ticket, cur, make_ticket(cur, nb, plen, clog));
         if (prev == cur) break;
         cur = prev;
     }
     unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
-    __hip_atomic_store(&A.ring[slot % A.ring_size], (slot << 32) | c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    st_c(&A.ring[slot % A.ring_size], (slot << 32) | c);
 }
 
 __device__ __forceinline__ bool round_drained(const SearchArgs& A, unsigned long long t) {
@@ -488,31 +503,31 @@ __device__ __forceinline__ void sys_store(T* p, T v) {
 // drained; its slot can then take a new cube.
 __device__ void post_done(const SearchArgs& A, uint32_t c) {
     SearchJob* J = &A.jobs[c];
-    fence_acquire();
     unsigned long long idx = atomicAdd(&A.q->done_next, 1ull);
     ServeDone* d = &A.done[idx % A.done_size];
-    uint32_t found = ld_relaxed(&J->found);
-    int32_t length = found ? (int32_t)J->length : -1;
+    uint32_t found = ld_c(&J->found);
+    uint32_t len = ld_c(&J->length);
+    int32_t length = found ? (int32_t)len : -1;
     unsigned long long nodes = 0;
-    uint32_t nr = J->n_rounds;
-    for (uint32_t r = 0; r < nr && r < (uint32_t)MAX_BOUND; ++r) nodes += J->round_nodes[r];
-    if (found && (nr == 0 || J->round_bound[nr - 1] != J->length)) nodes += ld_relaxed(&J->nodes);
-    sys_store(&d->tag, J->tag);
+    uint32_t nr = ld_c(&J->n_rounds);
+    for (uint32_t r = 0; r < nr && r < (uint32_t)MAX_BOUND; ++r) nodes += ld_c(&J->round_nodes[r]);
+    if (found && (nr == 0 || ld_c(&J->round_bound[nr - 1]) != len)) nodes += ld_c(&J->nodes);
+    sys_store(&d->tag, ld_c(&J->tag));
     sys_store(&d->slot, c);
     sys_store(&d->length, length);
     const uint32_t* sol = reinterpret_cast<const uint32_t*>(J->solution);
     uint32_t* dm = reinterpret_cast<uint32_t*>(d->moves);
-    for (int w = 0; w < MAX_BOUND / 4; ++w) sys_store(&dm[w], sol[w]);
+    for (int w = 0; w < MAX_BOUND / 4; ++w) sys_store(&dm[w], ld_c(&sol[w]));
     sys_store(&d->nodes, nodes);
-    sys_store(&d->t_admit, J->t_admit);
-    sys_store(&d->t_final, J->t_final);
+    sys_store(&d->t_admit, ld_c(&J->t_admit));
+    sys_store(&d->t_final, ld_c(&J->t_final));
     __hip_atomic_store(&d->seq, idx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __device__ void try_retire(const SearchArgs& A, uint32_t c) {
     SearchJob* J = &A.jobs[c];
-    if (!ld_relaxed(&J->found) && ld_relaxed(&J->status) != 2u) return;
-    unsigned long long t = ld_relaxed(&J->ticket);
+    if (!ld_c(&J->found) && ld_c(&J->status) != 2u) return;
+    unsigned long long t = ld_c(&J->ticket);
     while (t_inflight(t) == 0 && !(t & T_RETIRED)) {
         unsigned long long prev = atomicCAS(&J->ticket, t, t | T_RETIRED);
         if (prev == t) {
@@ -543,12 +558,18 @@ __device__ void release_items(const SearchArgs& A, uint32_t c, uint32_t n) {
 __device__ __forceinline__ void report_solution(const SearchArgs& A, SearchJob* J, bool queue, int g,
                                                 const uint8_t* pth, uint32_t stride) {
     if (atomicCAS(&J->found, 0u, 1u) != 0u) return;
-    J->length = (uint32_t)g;
-    for (int l = 0; l < g; ++l) J->solution[l] = pth[l * stride];
+    st_c(&J->length, (uint32_t)g);
+    uint32_t* sol = reinterpret_cast<uint32_t*>(J->solution);
+    for (int w = 0; w < MAX_BOUND / 4; ++w) {
+        uint32_t word = 0;
+        for (int b = 0; b < 4; ++b)
+            if (4 * w + b < g) word |= (uint32_t)pth[(4 * w + b) * stride] << (8 * b);
+        st_c(&sol[w], word);
+    }
     if (queue) {
-        J->status = 1;
-        J->t_final = wall_clock64();
-        fence_release();
+        st_c(&J->status, 1u);
+        st_c(&J->t_final, (unsigned long long)wall_clock64());
+        complete();
         atomicAdd(&A.q->n_final, 1u);
     }
 }
@@ -564,7 +585,7 @@ __device__ __forceinline__ uint32_t frame_lb_(const CubeLUT& L, const uint32_t* 
 // fetchers), and a ring entry.  Returns false when the mailbox is empty.
 template <int K, int F>
 __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
-    unsigned long long h = ld_relaxed(&A.q->mb_head);
+    unsigned long long h = ld_c(&A.q->mb_head);
     unsigned long long tail = __hip_atomic_load(A.mb_tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
     if (h >= tail) return false;
     if (atomicCAS(&A.q->mb_head, h, h + 1) != h) return true;  // another wave took it
@@ -575,31 +596,34 @@ __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
     for (int j = 0; j < 6; ++j) {
         const uint32_t* w = reinterpret_cast<const uint32_t*>(&r->frame[j]);
         Frame f{sys_load(&w[0]), sys_load(&w[1]), sys_load(&w[2]), sys_load(&w[3]), sys_load(&w[4])};
-        J->frame[j] = f;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&J->frame[j]);
+        st_c(&dst[0], f.c0);
+        st_c(&dst[1], f.c1);
+        st_c(&dst[2], f.e0);
+        st_c(&dst[3], f.e1);
+        st_c(&dst[4], f.e2);
         uint32_t v = frame_lb_<K, F>(L, A.classidx, A.table, f);
-        J->h[j] = v;
+        st_c(&J->h[j], v);
         bound = max(bound, v);
     }
-    J->tag = sys_load(&r->tag);
-    J->bound = bound;
-    J->length = 0;
-    J->n_rounds = 0;
-    J->t_admit = wall_clock64();
-    __hip_atomic_store(&J->nodes, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&J->status, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&J->found, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st_c(&J->tag, sys_load(&r->tag));
+    st_c(&J->bound, bound);
+    st_c(&J->length, 0u);
+    st_c(&J->n_rounds, 0u);
+    st_c(&J->t_admit, (unsigned long long)wall_clock64());
+    st_c(&J->nodes, 0ull);
+    st_c(&J->status, 0u);
+    st_c(&J->found, 0u);
     uint32_t plen = min(min(bound, (uint32_t)A.prefix_len), (uint32_t)MAX_PLEN);
-    fence_release();
-    unsigned long long cur = ld_relaxed(&J->ticket);
+    complete();
+    unsigned long long cur = ld_c(&J->ticket);
     for (;;) {
-        unsigned long long nt = (cur & T_INF_MASK) | ((unsigned long long)bound << T_BOUND_SHIFT) |
-                                ((unsigned long long)plen << T_PLEN_SHIFT);
-        unsigned long long prev = atomicCAS(&J->ticket, cur, nt);
+        unsigned long long prev = atomicCAS(&J->ticket, cur, make_ticket(cur, bound, plen, chunk_log_for(A, 0, A.pre_n[plen])));
         if (prev == cur) break;
         cur = prev;
     }
     unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
-    __hip_atomic_store(&A.ring[slot % A.ring_size], (slot << 32) | c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    st_c(&A.ring[slot % A.ring_size], (slot << 32) | c);
     return true;
 }
 
@@ -607,6 +631,22 @@ template <int K, int F>
 __device__ __forceinline__ uint32_t frame_lb(const CubeLUT& L, const uint32_t* __restrict__ classidx,
                                              const uint8_t* __restrict__ table, const Frame& f) {
     return table_lb<F>(table, table_index<K>(L, classidx, f));
+}
+
+// A lane finished its chunk of items: add its nodes to the round, wait for
+// that add, then give back its in-flight share of the ticket (the closer
+// reads the round's nodes).
+__device__ __forceinline__ void end_chunk(const SearchArgs& A, uint32_t c, SearchJob* J,
+                                          unsigned long long& chunk_nodes) {
+    if (chunk_nodes) atomicAdd(&J->nodes, chunk_nodes);
+    chunk_nodes = 0;
+    complete();
+    release_items(A, c, 1);
+}
+
+__device__ __forceinline__ Frame load_frame_c(const Frame* p) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+    return Frame{ld_c(&w[0]), ld_c(&w[1]), ld_c(&w[2]), ld_c(&w[3]), ld_c(&w[4])};
 }
 
 template <int K, int F, bool QUEUE>
@@ -626,7 +666,8 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
     SearchJob* J = nullptr;
     uint32_t bound = 0, todo = 0, ord = 0, rem = 0, iter = 0, idle = 0;
     int g = 0, base = 0, last = -1, m = -1, k = 0;
-    unsigned long long nodes = 0, probes = 0, all_nodes = 0, idle_since = 0;
+    unsigned long long nodes = 0, probes = 0, all_nodes = 0, idle_since = 0, chunk_nodes = 0;
+    uint32_t chunk_left = 0, item_cur = 0, cur_plen = 0;  // persistent form: rest of this lane's chunk
     bool have = false, done = false;
 
     for (;;) {
@@ -671,41 +712,53 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                     c = (uint32_t)A.active[w / A.n_prefix];
                     item = (uint32_t)(w % A.n_prefix);
                     J = &A.jobs[c];
-                    if (ld_relaxed(&J->found)) continue;
+                    if (ld_c(&J->found)) continue;
                     bound = J->bound;
                     plen = (uint32_t)A.plen;
+                } else if (chunk_left) {
+                    // next item of this lane's chunk: same cube and round, no atomics
+                    --chunk_left;
+                    item = ++item_cur;
+                    plen = cur_plen;
+                    if (ld_c(&J->found)) {
+                        chunk_left = 0;
+                        end_chunk(A, c, J, chunk_nodes);
+                        continue;
+                    }
                 } else {
                     if (idle && (--idle)) break;  // back off between empty fetches
                     unsigned long long mask = __ballot(1);
                     uint32_t cnt = __popcll(mask), rank = __popcll(mask & lanemask_lt());
-                    uint32_t cc = 0xFFFFFFFFu;
+                    uint32_t cc = 0xFFFFFFFFu, clog = 0;
                     unsigned long long old = 0;
                     if (rank == 0) {
                         for (int tries = 0; tries < 32; ++tries) {
-                            unsigned long long h = ld_acquire(&A.q->head), t = ld_acquire(&A.q->tail);
+                            unsigned long long h = ld_c(&A.q->head), t = ld_c(&A.q->tail);
                             if (h >= t) {
                                 if (A.serving && admit_one<K, F>(A, L)) continue;
                                 break;
                             }
-                            unsigned long long e = ld_acquire(&A.ring[h % A.ring_size]);
+                            unsigned long long e = ld_c(&A.ring[h % A.ring_size]);
                             if ((e >> 32) != (h & 0xFFFFFFFFull)) break;  // entry still being published
                             uint32_t ce = (uint32_t)e;
                             SearchJob* Je = &A.jobs[ce];
-                            unsigned long long tk = ld_acquire(&Je->ticket);
+                            unsigned long long tk = ld_c(&Je->ticket);
                             if ((tk & (T_CLOSING | T_RETIRED)) || t_next(tk) >= A.pre_n[t_plen(tk)] ||
-                                ld_relaxed(&Je->found)) {
+                                ld_c(&Je->found)) {
                                 atomicCAS(&A.q->head, h, h + 1);
                                 continue;
                             }
-                            old = atomicAdd(&Je->ticket, (unsigned long long)cnt * (T_INF_ONE + 1ull));
+                            clog = t_chunk(tk);
+                            old = atomicAdd(&Je->ticket, (unsigned long long)cnt * (T_INF_ONE + (1ull << clog)));
                             cc = ce;
                             break;
                         }
                     }
                     cc = first_lane_u32(cc);
                     old = first_lane_u64(old);
+                    clog = first_lane_u32(clog);
                     if (cc == 0xFFFFFFFFu) {
-                        if (ld_relaxed(&A.q->n_final) >= A.q->n_jobs || ld_relaxed(&A.q->stop) ||
+                        if (ld_c(&A.q->n_final) >= A.q->n_jobs || ld_c(&A.q->stop) ||
                             (A.serving && sys_load(A.host_stop))) {
                             done = true;
                         } else {
@@ -718,27 +771,38 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                         break;
                     }
                     idle_since = 0;
-                    item = t_next(old) + rank;
                     plen = t_plen(old);
-                    bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < A.pre_n[plen];
+                    const uint32_t n_items = A.pre_n[plen];
+                    item = t_next(old) + (rank << clog);
+                    bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < n_items;
                     unsigned long long inval = __ballot(!valid);
                     if (rank == 0 && inval) release_items(A, cc, __popcll(inval));
                     if (!valid) continue;
                     c = cc;
                     J = &A.jobs[c];
                     bound = t_bound(old);
-                    fence_acquire();  // the job's frames were published before its ticket
-                    if (ld_relaxed(&J->found)) {
-                        release_items(A, c, 1);
+                    cur_plen = plen;
+                    item_cur = item;
+                    chunk_left = min(item + (1u << clog), n_items) - item - 1u;
+                    chunk_nodes = 0;
+                    if (ld_c(&J->found)) {
+                        chunk_left = 0;
+                        end_chunk(A, c, J, chunk_nodes);
                         continue;
                     }
                 }
                 // ---- replay the prefix with all six frames (kernels.cu:1552-1614) ----
                 const uint8_t* pre = (QUEUE ? A.pre_tab[plen] : A.prefixes) + (unsigned long long)item * plen;
-                static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame(&J->frame[j]); });
                 hp = 0;
+                if (QUEUE && A.serving) {  // the slot may have held another cube: read past the L2
+                    static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame_c(&J->frame[j]); });
 #pragma unroll
-                for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
+                    for (int j = 0; j < 6; ++j) hp |= ld_c(&J->h[j]) << (5 * j);
+                } else {
+                    static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame(&J->frame[j]); });
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
+                }
                 nodes = 0;
                 g = 0;
                 last = -1;
@@ -770,10 +834,11 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                 }
                 if (!ok) {
                     all_nodes += nodes;
-                    atomicAdd(&J->nodes, nodes);
                     if (QUEUE) {
-                        fence_release();
-                        release_items(A, c, 1);
+                        chunk_nodes += nodes;
+                        if (!chunk_left) end_chunk(A, c, J, chunk_nodes);
+                    } else {
+                        atomicAdd(&J->nodes, nodes);
                     }
                     continue;
                 }
@@ -788,7 +853,7 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                 have = true;
                 continue;
             }
-            if (todo && !(((++iter & A.poll_mask) == 0) && ld_relaxed(&J->found))) {
+            if (todo && !(((++iter & A.poll_mask) == 0) && ld_c(&J->found))) {
                 // ---- pick the next canonical child ----
                 m = (int)ctz(todo);
                 todo &= todo - 1;
@@ -801,12 +866,16 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                 break;
             }
             // ---- node exhausted: pop, or finish the item ----
-            if (g == base || ld_relaxed(&J->found)) {
+            if (g == base || ld_c(&J->found)) {
                 all_nodes += nodes;
-                atomicAdd(&J->nodes, nodes);
                 if (QUEUE) {
-                    fence_release();
-                    release_items(A, c, 1);
+                    chunk_nodes += nodes;
+                    if (!chunk_left || ld_c(&J->found)) {
+                        chunk_left = 0;
+                        end_chunk(A, c, J, chunk_nodes);
+                    }
+                } else {
+                    atomicAdd(&J->nodes, nodes);
                 }
                 have = false;
                 continue;

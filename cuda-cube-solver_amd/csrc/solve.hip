@@ -162,6 +162,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     A.deeper0 = d0;
     A.deeper1 = d1;
     A.poll_mask = (unsigned)o.poll_interval - 1u;
+    A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : 4096ull;
     A.stats = d_stats;
     for (int p = 0; p <= MAX_PLEN; ++p) {
         A.pre_tab[p] = t->d_prefix[p];
@@ -261,7 +262,10 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             lj[k] = jobs[live[k]];
             uint32_t b = lj[k].bound;
             uint32_t plen = std::min(std::min(b, (uint32_t)o.prefix_len), (uint32_t)MAX_PLEN);
-            lj[k].ticket = ((unsigned long long)b << 49) | ((unsigned long long)plen << 54);
+            // first round: chunks of up to 64 items, at least 4096 chunks per round
+            unsigned long long clog = 0;
+            while (clog < 6 && (t->prefix_n[plen] >> (clog + 1)) >= 4096u) ++clog;
+            lj[k].ticket = ((unsigned long long)b << 49) | ((unsigned long long)plen << 54) | (clog << 59);
         }
         const unsigned long long ring_size = 2ull * nl + 1024;
         std::vector<unsigned long long> ring(ring_size, ~0ull);

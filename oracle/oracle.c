@@ -787,10 +787,16 @@ static int allowed(int last, int m) { /* solver.py:33-37 */
 
 static int inverse_move(int m) { return 3 * (m / 3) + 2 - m % 3; }
 
+/* Wall-clock deadline for bounded timing runs (0 = none): a search past it
+ * stops and reports the nodes it visited. */
+static double g_deadline = 0.0;
+void or_set_deadline(double seconds_from_now) { g_deadline = seconds_from_now > 0 ? omp_get_wtime() + seconds_from_now : 0.0; }
+
 typedef struct {
     const okind* k;
     const uint8_t* table;
     int fmt;
+    int expired;
     int frame_move[3][18];
     uint32_t bound;
     uint64_t nodes;
@@ -825,7 +831,8 @@ static int all_axes_tight(const uint32_t* c, uint32_t budget) { /* kernels.cu:76
 static void dfs(sctx* S, ocube* f, const uint32_t* h, uint32_t g, int last) {
     uint32_t bound = S->bound;
     uint32_t excl = excluded_axes(h, bound - g);
-    for (int m = 0; m < 18 && !S->found; ++m) {
+    if (g_deadline > 0 && (S->nodes & 1023u) == 0 && omp_get_wtime() > g_deadline) S->expired = 1;
+    for (int m = 0; m < 18 && !S->found && !S->expired; ++m) {
         if (!allowed(last, m)) continue;
         if ((excl >> ((m / 3) % 3)) & 1u) continue;
         S->nodes++;
@@ -870,7 +877,7 @@ static void run_items(sctx* S, const ocube* root_f, const uint32_t* root_h, int 
     seq[0] = -1;
     /* iterative lexicographic enumeration */
     for (;;) {
-        if (S->found) return;
+        if (S->found || S->expired) return;
         /* advance seq[depth] to the next allowed move */
         int last = depth ? seq[depth - 1] : -1;
         int m = seq[depth] + 1;
@@ -979,6 +986,7 @@ int or_solve(const okind* k, const uint8_t* table, int fmt, const ocube* cube, c
         res->n_rounds++;
         res->nodes += S.nodes;
         last = S.nodes;
+        if (S.expired) return -2;
         if (S.found) {
             res->length = (int)bound;
             memcpy(res->moves, S.solution, bound);

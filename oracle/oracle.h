@@ -83,6 +83,8 @@ int or_solve(const okind* k, const uint8_t* table, int fmt, const ocube* cube, c
              osolve_result* res);
 int or_solve_many(const okind* k, const uint8_t* table, int fmt, const ocube* cubes, int n,
                   const osolve_opts* opts, osolve_result* res, int n_threads, double* seconds);
+/* Stop searches after this many seconds of wall time (0 = no limit). */
+void or_set_deadline(double seconds_from_now);
 /* Canonical move sequences of a length (solver.py:33-48). */
 int64_t or_canonical_prefixes(int length, uint8_t* out, int64_t cap);
 

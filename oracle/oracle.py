@@ -69,6 +69,7 @@ def lib() -> ctypes.CDLL:
         L.or_solve_many.argtypes = [P, P, ctypes.c_int, P, ctypes.c_int, P, P, ctypes.c_int, P]
         L.or_canonical_prefixes.restype = ctypes.c_int64
         L.or_canonical_prefixes.argtypes = [ctypes.c_int, P, ctypes.c_int64]
+        L.or_set_deadline.argtypes = [ctypes.c_double]
         L.or_sym_conjugate.argtypes = [ctypes.c_int, P, P]
         L.or_sym_matrix.argtypes = [ctypes.c_int, P]
         L.or_axis_frames.argtypes = [P, P]
@@ -231,6 +232,19 @@ def solve_many(k: OKind, table: np.ndarray, fmt: int, arr: np.ndarray, prefix_le
                     "rounds": [(int(r.round_bound[j]), int(r.round_nodes[j])) for j in range(r.n_rounds)],
                     "nodes": int(r.nodes), "root_h": list(r.root_h), "seconds": float(secs[i])})
     return out, bad
+
+
+def timed_nodes(k: OKind, table: np.ndarray, fmt: int, arr: np.ndarray, seconds: float, n_threads: int):
+    """Nodes the oracle visits on these cubes (one per thread) within a wall-clock budget."""
+    import time
+
+    lib().or_set_deadline(seconds)
+    t = time.perf_counter()
+    try:
+        res, _ = solve_many(k, table, fmt, arr, n_threads=n_threads)
+    finally:
+        lib().or_set_deadline(0.0)
+    return sum(r["nodes"] for r in res), time.perf_counter() - t, res
 
 
 def complete_round_nodes(result: dict) -> int:
