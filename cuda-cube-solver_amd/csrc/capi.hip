@@ -47,15 +47,18 @@ int tables_setup(ccs_tables* t, const std::string& name) {
 int ensure_prefixes(ccs_tables* t, int plen) {
     if (plen < 0 || plen > MAX_PLEN) return fail(CCS_EINVAL, "prefix length out of range");
     if (t->prefix_n[plen]) return CCS_OK;
-    if (plen == 0) {
-        HIP_TRY(hipMalloc(&t->d_prefix[0], 16));
-        t->prefix_n[0] = 1;
-        return CCS_OK;
+    // one item per 64-bit word, move j in byte j (zero past the prefix)
+    std::vector<unsigned long long> words(1, 0ull);
+    if (plen > 0) {
+        std::vector<uint8_t> pre = canonical_prefixes(plen);
+        const size_t n = pre.size() / plen;
+        words.assign(n, 0ull);
+        for (size_t i = 0; i < n; ++i)
+            for (int j = 0; j < plen; ++j) words[i] |= (unsigned long long)pre[i * plen + j] << (8 * j);
     }
-    std::vector<uint8_t> pre = canonical_prefixes(plen);
-    HIP_TRY(hipMalloc(&t->d_prefix[plen], pre.size()));
-    HIP_TRY(hipMemcpy(t->d_prefix[plen], pre.data(), pre.size(), hipMemcpyHostToDevice));
-    t->prefix_n[plen] = (uint32_t)(pre.size() / plen);
+    HIP_TRY(hipMalloc(&t->d_prefix[plen], words.size() * 8));
+    HIP_TRY(hipMemcpy(t->d_prefix[plen], words.data(), words.size() * 8, hipMemcpyHostToDevice));
+    t->prefix_n[plen] = (uint32_t)words.size();
     return CCS_OK;
 }
 
@@ -377,6 +380,35 @@ int ccs_unrank_check(ccs_tables_t* t, const uint64_t* idx, int n, uint64_t* out)
     HIP_TRY(hipMemcpy(out, d_out, 8ull * n, hipMemcpyDeviceToHost));
     hipFree(d_idx);
     hipFree(d_out);
+    return CCS_OK;
+}
+
+int ccs_gather_bench(ccs_tables_t* t, int mode, uint64_t n_probes, int blocks_per_cu, double* ms,
+                     uint64_t* checksum) {
+    if (!t || mode < 0 || mode > 2 || !ms) return fail(CCS_EINVAL, "bad arguments");
+    int fmt = t->format < 0 ? 0 : t->format;
+    if (!t->d_table[fmt]) return fail(CCS_EINVAL, "no table resident");
+    unsigned long long* d_sum = nullptr;
+    HIP_TRY(hipMalloc(&d_sum, 8));
+    HIP_TRY(hipMemsetAsync(d_sum, 0, 8, t->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int grid = device_cus() * (blocks_per_cu > 0 ? blocks_per_cu : 8);
+    HIP_TRY(hipEventRecord(e0, t->stream));
+    HIP_TRY(launch_gather(mode, grid, reinterpret_cast<const uint32_t*>(t->d_table[fmt]), t->table_bytes[fmt] / 32,
+                          n_probes, d_sum, t->stream));
+    HIP_TRY(hipEventRecord(e1, t->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float f = 0;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    unsigned long long sum = 0;
+    HIP_TRY(hipMemcpy(&sum, d_sum, 8, hipMemcpyDeviceToHost));
+    if (checksum) *checksum = sum;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d_sum);
     return CCS_OK;
 }
 

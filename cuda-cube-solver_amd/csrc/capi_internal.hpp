@@ -63,7 +63,7 @@ struct ccs_tables {
     uint16_t* d_class_stab = nullptr;
     uint8_t* d_table[2] = {nullptr, nullptr};
     uint64_t table_bytes[2] = {0, 0};
-    uint8_t* d_prefix[ccs::MAX_PLEN + 1] = {};
+    unsigned long long* d_prefix[ccs::MAX_PLEN + 1] = {};  // packed prefixes, 8 B per item
     uint32_t prefix_n[ccs::MAX_PLEN + 1] = {};
     hipStream_t stream = nullptr;
 

@@ -560,22 +560,37 @@ void build_lut(const Kind& k, CubeLUT& L) {
     Statics& S = st();
     memset(&L, 0, sizeof L);
     auto put = [](uint32_t& w, int j, uint32_t b) { w |= (b & 0xFFu) << (8 * j); };
-    for (int m = 0; m < 18; ++m) {
+    auto right_rec = [&](RightRec& r, int m) {
         const Cubie& mv = S.moves[m];
         for (int w = 0; w < 2; ++w)
             for (int j = 0; j < 4; ++j) {
-                put(L.mc_sel[m][w], j, mv.cp[4 * w + j]);
-                put(L.mc_add[m][w], j, (uint32_t)mv.co[4 * w + j] << 3);
-                put(L.lc[m][w], j, mv.cp[4 * w + j] | mv.co[4 * w + j] << 3);
+                put(r.csel[w], j, mv.cp[4 * w + j]);
+                put(r.cadd[w], j, (uint32_t)mv.co[4 * w + j] << 3);
             }
         for (int w = 0; w < 3; ++w)
             for (int j = 0; j < 4; ++j) {
                 uint32_t src = mv.ep[4 * w + j];
-                put(L.me_lo[m][w], j, src < 8 ? src : SEL_ZERO);
-                put(L.me_hi[m][w], j, src >= 8 ? src - 8 : SEL_ZERO);
-                put(L.me_flip[m][w], j, (uint32_t)mv.eo[4 * w + j] << 4);
-                put(L.le[m][w], j, mv.ep[4 * w + j] | mv.eo[4 * w + j] << 4);
+                put(r.elo[w], j, src < 8 ? src : SEL_ZERO);
+                put(r.ehi[w], j, src >= 8 ? src - 8 : SEL_ZERO);
+                put(r.eflip[w], j, (uint32_t)mv.eo[4 * w + j] << 4);
             }
+    };
+    auto left_rec = [&](LeftRec& r, int m) {
+        const Cubie& mv = S.moves[m];
+        for (int w = 0; w < 2; ++w)
+            for (int j = 0; j < 4; ++j) put(r.lc[w], j, mv.cp[4 * w + j] | mv.co[4 * w + j] << 3);
+        for (int w = 0; w < 3; ++w)
+            for (int j = 0; j < 4; ++j) put(r.le[w], j, mv.ep[4 * w + j] | mv.eo[4 * w + j] << 4);
+    };
+    auto inverse = [](int m) { return 3 * (m / 3) + 2 - m % 3; };
+    for (int m = 0; m < 18; ++m) {
+        const int fm[3] = {m, S.axis_map[m], S.axis_map[S.axis_map[m]]};  // m as axis frame j sees it
+        for (int j = 0; j < 3; ++j) {
+            right_rec(L.rr[0][j][m], fm[j]);
+            right_rec(L.rr[1][j][m], inverse(fm[j]));
+            left_rec(L.lr[0][j][m], inverse(fm[j]));
+            left_rec(L.lr[1][j][m], fm[j]);
+        }
     }
     for (int s = 0; s < 16; ++s) {
         const SymAction& y = S.ud[s];
@@ -622,11 +637,8 @@ void build_lut(const Kind& k, CubeLUT& L) {
     memcpy(L.gen_class, k.gen_class, sizeof L.gen_class);
     memcpy(L.gen_rep, k.gen_rep, sizeof L.gen_rep);
     memcpy(L.s_mul, S.s_mul, sizeof L.s_mul);
-    for (int m = 0; m < 18; ++m) {
-        L.frame_move[0][m] = (uint8_t)m;
-        L.frame_move[1][m] = (uint8_t)S.axis_map[m];
-        L.frame_move[2][m] = (uint8_t)S.axis_map[S.axis_map[m]];
-    }
+    for (int b = 0; b < 243; ++b)
+        for (int k = 0, v = b; k < 5; ++k, v /= 3) L.b3digits[b] |= (uint16_t)((v % 3) << (2 * k));
     for (int last = -1; last < 18; ++last) {
         uint32_t mask = 0;
         for (int m = 0; m < 18; ++m)

@@ -51,50 +51,79 @@ CCS_HD uint32_t ctz(uint32_t x) { return (uint32_t)__builtin_ctz(x); }
 CCS_HD uint32_t shl(uint32_t x, uint32_t s) { return s >= 32u ? 0u : x << s; }
 CCS_HD uint64_t shl64(uint64_t x, uint32_t s) { return s >= 64u ? 0u : x << s; }
 
-// Bit `bit` of each byte -> 4-bit mask (byte 0 -> bit 0).
-CCS_HD uint32_t gather4(uint32_t v, int bit) { return (((v >> bit) & 0x01010101u) * 0x01020408u) >> 24; }
+// Sum of the four byte products of a and b, plus c (v_dot4_u32_u8, full rate;
+// 32-bit multiplies are quarter rate).
+CCS_HD uint32_t dot4(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_udot4(a, b, c, false);
+#else
+    for (int i = 0; i < 4; ++i) c += ((a >> (8 * i)) & 0xFFu) * ((b >> (8 * i)) & 0xFFu);
+    return c;
+#endif
+}
+// Product of two values below 2^24 (v_mul_u32_u24, full rate).
+CCS_HD uint32_t mul24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
+}
 
-// Corner twist stored in bits 3..5 of each byte may reach 4 after an add: reduce mod 3.
-CCS_HD uint32_t twist_mod3(uint32_t x) { return x - (((x + 0x08080808u) >> 5) & 0x01010101u) * 0x18u; }
+// Bit `bit` of each byte -> 4-bit mask (byte 0 -> bit 0).
+CCS_HD uint32_t gather4(uint32_t v, int bit) { return dot4((v >> bit) & 0x01010101u, 0x08040201u, 0u); }
+
+// Corner twist stored in bits 3..5 of each byte may reach 4 after an add:
+// reduce mod 3 by taking 0x18 off the bytes whose twist is 3 or 4 (s marks
+// them with 0x20 per byte, and 0x20 - 0x08 = 0x18).
+CCS_HD uint32_t twist_mod3(uint32_t x) {
+    uint32_t s = (x + 0x08080808u) & 0x20202020u;
+    return x - s + (s >> 2);
+}
 
 // Relabel edge cubies of a word through a 12-byte table (rel0..rel2): new
-// byte = rel[cubie] with the existing flip bit xored in.
+// byte = rel[cubie] with the existing flip bit xored in.  Table bytes are
+// below 0x80, so selectors 8..11 (sign fill) read as zero: cubie c selects
+// c from rel0:rel1 (8..11 -> zero) and (c & 11) ^ 8 from rel2 (0..7 -> 8..11
+// -> zero, 8..11 -> 0..3).
 CCS_HD uint32_t relabel_edges(uint32_t v, uint32_t r0, uint32_t r1, uint32_t r2) {
-    uint32_t sel = v & 0x0F0F0F0Fu;
-    uint32_t x = vperm(r1, r0, sel);                 // cubies 0..7 (8..11 read as zero)
-    uint32_t y = vperm(r2, r2, sel & 0x03030303u);   // cubies 8..11
-    uint32_t hi = ((v >> 3) & 0x01010101u) * 0xFFu;
-    return (x | (y & hi)) ^ (v & 0x10101010u);
+    uint32_t x = vperm(r1, r0, v & 0x0F0F0F0Fu);                  // cubies 0..7
+    uint32_t y = vperm(r2, r2, (v & 0x0B0B0B0Bu) ^ 0x08080808u);  // cubies 8..11
+    return (x | y) ^ (v & 0x10101010u);
 }
 
-// state then move m (right multiplication; kernels.cu:84-101 computes the same).
-CCS_HD void apply_move(const CubeLUT& L, Frame& f, int m) {
-    uint32_t a = twist_mod3(vperm(f.c1, f.c0, L.mc_sel[m][0]) + L.mc_add[m][0]);
-    uint32_t b = twist_mod3(vperm(f.c1, f.c0, L.mc_sel[m][1]) + L.mc_add[m][1]);
-    uint32_t e0 = (vperm(f.e1, f.e0, L.me_lo[m][0]) | vperm(f.e2, f.e2, L.me_hi[m][0])) ^ L.me_flip[m][0];
-    uint32_t e1 = (vperm(f.e1, f.e0, L.me_lo[m][1]) | vperm(f.e2, f.e2, L.me_hi[m][1])) ^ L.me_flip[m][1];
-    uint32_t e2 = (vperm(f.e1, f.e0, L.me_lo[m][2]) | vperm(f.e2, f.e2, L.me_hi[m][2])) ^ L.me_flip[m][2];
+// state then move (right multiplication; kernels.cu:84-101 computes the same).
+CCS_HD void right_mul(Frame& f, const RightRec& r) {
+    uint32_t a = twist_mod3(vperm(f.c1, f.c0, r.csel[0]) + r.cadd[0]);
+    uint32_t b = twist_mod3(vperm(f.c1, f.c0, r.csel[1]) + r.cadd[1]);
+    uint32_t e0 = (vperm(f.e1, f.e0, r.elo[0]) | vperm(f.e2, f.e2, r.ehi[0])) ^ r.eflip[0];
+    uint32_t e1 = (vperm(f.e1, f.e0, r.elo[1]) | vperm(f.e2, f.e2, r.ehi[1])) ^ r.eflip[1];
+    uint32_t e2 = (vperm(f.e1, f.e0, r.elo[2]) | vperm(f.e2, f.e2, r.ehi[2])) ^ r.eflip[2];
     f = Frame{a, b, e0, e1, e2};
 }
 
-// move m then state (left multiplication; used by the inverse frames).
-CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) {
-    uint32_t a = twist_mod3(vperm(L.lc[m][1], L.lc[m][0], f.c0 & 0x07070707u) + (f.c0 & 0x18181818u));
-    uint32_t b = twist_mod3(vperm(L.lc[m][1], L.lc[m][0], f.c1 & 0x07070707u) + (f.c1 & 0x18181818u));
-    uint32_t e0 = relabel_edges(f.e0, L.le[m][0], L.le[m][1], L.le[m][2]);
-    uint32_t e1 = relabel_edges(f.e1, L.le[m][0], L.le[m][1], L.le[m][2]);
-    uint32_t e2 = relabel_edges(f.e2, L.le[m][0], L.le[m][1], L.le[m][2]);
+// move then state (left multiplication; used by the inverse frames).
+CCS_HD void left_mul(Frame& f, const LeftRec& r) {
+    uint32_t a = twist_mod3(vperm(r.lc[1], r.lc[0], f.c0 & 0x07070707u) + (f.c0 & 0x18181818u));
+    uint32_t b = twist_mod3(vperm(r.lc[1], r.lc[0], f.c1 & 0x07070707u) + (f.c1 & 0x18181818u));
+    uint32_t e0 = relabel_edges(f.e0, r.le[0], r.le[1], r.le[2]);
+    uint32_t e1 = relabel_edges(f.e1, r.le[0], r.le[1], r.le[2]);
+    uint32_t e2 = relabel_edges(f.e2, r.le[0], r.le[1], r.le[2]);
     f = Frame{a, b, e0, e1, e2};
 }
+
+CCS_HD void apply_move(const CubeLUT& L, Frame& f, int m) { right_mul(f, L.rr[0][0][m]); }
+CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) { left_mul(f, L.lr[1][0][m]); }
 
 CCS_HD int inverse_move(int m) { return 3 * (m / 3) + 2 - m % 3; }
 
 // twist (3^7) * 70 + rank of the D-corner positions (coords.py:112-118).
 CCS_HD uint32_t corner_raw(const CubeLUT& L, const Frame& f) {
     uint32_t olo = (f.c0 >> 3) & 0x03030303u, ohi = (f.c1 >> 3) & 0x00030303u;
-    uint32_t twist = ((olo * 0x0103091Bu) >> 24) + 81u * ((ohi * 0x01030900u) >> 24);
-    uint32_t dmask = gather4(f.c0, 2) | (gather4(f.c1, 2) << 4);
-    return twist * 70u + L.ccomb_rank[dmask];
+    // sum of co[i] * 3^i over corners 0..6 (corner 7 is fixed by the others)
+    uint32_t twist = dot4(olo, 0x1B090301u, mul24(dot4(ohi, 0x00090301u, 0u), 81u));
+    uint32_t dmask = dot4((f.c0 >> 2) & 0x01010101u, 0x08040201u, dot4((f.c1 >> 2) & 0x01010101u, 0x80402010u, 0u));
+    return mul24(twist, 70u) + L.ccomb_rank[dmask];
 }
 
 // K = 0: the 304 kind; K > 0: a group kind with K E-content classes.
@@ -128,16 +157,16 @@ CCS_HD uint32_t edge_coord(const CubeLUT& L, const Frame& f, uint32_t s) {
                   shl64(vperm(e[1], e[1], L.csel[m1 ^ 15u]), 8u * k0) |
                   shl64(vperm(e[2], e[2], L.csel[m2 ^ 15u]), 8u * (k0 + k1));
     uint32_t nlo = (uint32_t)nw, nhi = (uint32_t)(nw >> 32);
-    if (K == 0) {
+    if constexpr (K == 0) {
         // U-layer edges are cubies 0..3 (bits 2 and 3 clear)
         uint32_t umask = gather4(~(nlo | (nlo >> 1)), 2) | (gather4(~(nhi | (nhi >> 1)), 2) << 4);
-        return (slice * 70u + L.ccomb_rank[umask]) * 16u + eflip;
+        return (mul24(slice, 70u) + L.ccomb_rank[umask]) * 16u + eflip;
     } else {
         uint32_t lab = ew & 0x03030303u;
         uint32_t key = (lab | (lab >> 6) | (lab >> 12) | (lab >> 18)) & 0xFFu;
         uint32_t nflip = gather4(nlo, 4) | (gather4(nhi, 4) << 4);
         uint32_t c = L.gen_class[(uint32_t)L.eorder_rank[key] * 16u + eflip];
-        return (slice * 256u + nflip) * (uint32_t)K + c;
+        return mul24(slice * 256u + nflip, (uint32_t)K) + c;
     }
 }
 
@@ -154,7 +183,7 @@ CCS_HD uint64_t table_index(const CubeLUT& L, const uint32_t* __restrict__ class
 // (header m | c << 6, 155 digits).
 template <int F>
 CCS_HD uint32_t table_lb(const uint8_t* __restrict__ tbl, uint64_t idx) {
-    if (F == 0) {
+    if constexpr (F == 0) {
         uint64_t b = idx / 124u;
         uint32_t pos = 8u + 2u * (uint32_t)(idx - b * 124u);
         const uint32_t* p = reinterpret_cast<const uint32_t*>(tbl + b * 32u);
@@ -166,6 +195,34 @@ CCS_HD uint32_t table_lb(const uint8_t* __restrict__ tbl, uint64_t idx) {
         uint32_t hdr = p[0], byte = p[1u + j / 5u], k = j % 5u;
         uint32_t d = (byte / (k == 0 ? 1u : k == 1 ? 3u : k == 2 ? 9u : k == 3 ? 27u : 81u)) % 3u;
         uint32_t m = hdr & 63u;
+        return d == 0u ? m : m + (hdr >> 6) + d;
+    }
+}
+
+// Lower bound of a frame from the solving table, with the block number and
+// the entry within the block computed in 32-bit arithmetic: with
+// N_EDGE = Q * D + R, index / D = class * Q + (class * R + edge) / D, and the
+// bracket stays below 2^23 (class * N_EDGE itself reaches 2^35).
+template <int K, int F>
+CCS_HD uint32_t frame_bound(const CubeLUT& L, const uint32_t* __restrict__ classidx,
+                            const uint8_t* __restrict__ tbl, const Frame& f) {
+    constexpr uint32_t D = F == 0 ? 124u : 155u;
+    constexpr uint32_t Q = (uint32_t)(KindT<K>::N_EDGE / D), R = (uint32_t)(KindT<K>::N_EDGE % D);
+    static_assert(Q < (1u << 24), "class * Q must stay a 24-bit product");
+    const uint32_t packed = classidx[corner_raw(L, f)];
+    const uint32_t cls = packed >> 4;
+    const uint32_t x = mul24(cls, R) + edge_coord<K>(L, f, packed & 15u);
+    const uint32_t q = x / D;
+    const uint32_t j = x - mul24(q, D);
+    const uint8_t* p = tbl + (uint64_t)(mul24(cls, Q) + q) * 32u;
+    if constexpr (F == 0) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+        const uint32_t pos = 8u + 2u * j;
+        return (w[0] & 0xFFu) + ((w[pos >> 5] >> (pos & 31u)) & 3u);
+    } else {
+        const uint32_t hdr = p[0], byte = p[1u + j / 5u];
+        const uint32_t d = (L.b3digits[byte] >> (2u * (j % 5u))) & 3u;
+        const uint32_t m = hdr & 63u;
         return d == 0u ? m : m + (hdr >> 6) + d;
     }
 }
@@ -193,7 +250,7 @@ CCS_HD Frame unrank(const CubeLUT& L, const uint32_t* __restrict__ class_rep, ui
         c[i >> 2] |= (cp | (o << 3)) << ((i & 3) * 8);
     }
     uint32_t emask, eord, umask, eflips, nflips;
-    if (K == 0) {
+    if constexpr (K == 0) {
         uint32_t eo = ec % 16u, rest = ec / 16u;
         emask = L.slice_unrank[rest / 70u];
         umask = L.ccomb_unrank[rest % 70u];

@@ -410,12 +410,12 @@ __device__ __forceinline__ Frame load_frame(const Frame* p) {
     return Frame{w[0], w[1], w[2], w[3], w[4]};
 }
 __device__ __forceinline__ void frame_do(const CubeLUT& L, Frame& f, int j, int m) {
-    if (j < 3) apply_move(L, f, L.frame_move[j][m]);
-    else apply_left(L, f, inverse_move(L.frame_move[j - 3][m]));
+    if (j < 3) right_mul(f, L.rr[0][j][m]);
+    else left_mul(f, L.lr[0][j - 3][m]);
 }
 __device__ __forceinline__ void frame_undo(const CubeLUT& L, Frame& f, int j, int m) {
-    if (j < 3) apply_move(L, f, inverse_move(L.frame_move[j][m]));
-    else apply_left(L, f, L.frame_move[j - 3][m]);
+    if (j < 3) right_mul(f, L.rr[1][j][m]);
+    else left_mul(f, L.lr[1][j - 3][m]);
 }
 
 // Ticket of the persistent form: one 64-bit word per cube packs the round
@@ -577,7 +577,7 @@ __device__ __forceinline__ void report_solution(const SearchArgs& A, SearchJob* 
 template <int K, int F>
 __device__ __forceinline__ uint32_t frame_lb_(const CubeLUT& L, const uint32_t* __restrict__ classidx,
                                               const uint8_t* __restrict__ table, const Frame& f) {
-    return table_lb<F>(table, table_index<K>(L, classidx, f));
+    return frame_bound<K, F>(L, classidx, table, f);
 }
 
 // Serving: take one mailbox record into its slot: root bounds of the six
@@ -630,7 +630,7 @@ __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
 template <int K, int F>
 __device__ __forceinline__ uint32_t frame_lb(const CubeLUT& L, const uint32_t* __restrict__ classidx,
                                              const uint8_t* __restrict__ table, const Frame& f) {
-    return table_lb<F>(table, table_index<K>(L, classidx, f));
+    return frame_bound<K, F>(L, classidx, table, f);
 }
 
 // A lane finished its chunk of items: add its nodes to the round, wait for
@@ -650,24 +650,31 @@ __device__ __forceinline__ Frame load_frame_c(const Frame* p) {
 }
 
 template <int K, int F, bool QUEUE>
-__global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
+__global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kernel(SearchArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     CubeLUT& L = *reinterpret_cast<CubeLUT*>(smem);
-    const uint32_t nt = blockDim.x, tid = threadIdx.x;
+    constexpr uint32_t nt = SEARCH_WG;  // the host launches exactly this many lanes
+    const uint32_t tid = threadIdx.x;
     uint32_t* hst = reinterpret_cast<uint32_t*>(smem + sizeof(CubeLUT)) + tid;
-    uint8_t* pth = reinterpret_cast<uint8_t*>(reinterpret_cast<uint32_t*>(smem + sizeof(CubeLUT)) + MAX_REL * nt) + tid;
+    uint8_t* pth = reinterpret_cast<uint8_t*>(reinterpret_cast<uint32_t*>(smem + sizeof(CubeLUT)) + STACK_DEPTH * nt) + tid;
     load_lut(&L, A.lut);
     const uint32_t* __restrict__ classidx = A.classidx;
     const uint8_t* __restrict__ table = A.table;
 
     Frames6 f;
     uint32_t hp = 0, cp = 0;
-    uint32_t c = 0;  // cube of the current item
+    uint32_t c = 0xFFFFFFFFu;  // cube of the current item
     SearchJob* J = nullptr;
-    uint32_t bound = 0, todo = 0, ord = 0, rem = 0, iter = 0, idle = 0;
-    int g = 0, base = 0, last = -1, m = -1, k = 0;
+    uint32_t bound = 0, todo = 0, ord = 0, rem = 0, iter = 0, idle = 0, plen = 0;
+    int g = 0, tgt = 0, last = -1, m = -1, k = 0;
     unsigned long long nodes = 0, probes = 0, all_nodes = 0, idle_since = 0, chunk_nodes = 0;
-    uint32_t chunk_left = 0, item_cur = 0, cur_plen = 0;  // persistent form: rest of this lane's chunk
+    unsigned long long pbits = 0;  // the item's prefix, move j in byte j
+    uint32_t chunk_left = 0, item_cur = 0;  // persistent form: rest of this lane's chunk
+    // The lane's frames stay at depth `hold` on the previous item's prefix
+    // (-1: nothing held); `hfail` is the depth of that item's failing prefix
+    // move (-1: none).  The next item of the same cube and round undoes back
+    // to the prefix it shares and replays only the rest.
+    int hold = -1, hfail = -1;
     bool have = false, done = false;
 
     for (;;) {
@@ -679,8 +686,8 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                     break;
                 }
                 if (k == 6 && !all_tight(cp, bound - (uint32_t)g - 1u)) {  // survived: push
-                    static_for<0, 6>([&](auto J) { frame_do(L, f.template at<J>(), J, m); });
-                    hst[(g - base) * nt] = hp;
+                    static_for<0, 6>([&](auto j) { frame_do(L, f.template at<j>(), j, m); });
+                    hst[g * nt] = hp;
                     pth[g * nt] = (uint8_t)m;
                     hp = cp;
                     last = m;
@@ -692,13 +699,33 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                         todo = L.amask[m + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
                         ord = frame_order(hp, bound - (uint32_t)g);
                     }
+                } else if ((uint32_t)g < plen) {  // a prefix move was pruned: the item ends
+                    hfail = g;
+                    tgt = -1;
                 }
                 m = -1;
                 continue;
             }
+            if (tgt < 0) {
+                // ---- the item is finished: account its nodes ----
+                hold = (uint32_t)g <= plen ? g : -1;  // deeper only when found ended the search
+                all_nodes += nodes;
+                if (QUEUE) {
+                    chunk_nodes += nodes;
+                    if (!chunk_left || ld_c(&J->found)) {
+                        chunk_left = 0;
+                        end_chunk(A, c, J, chunk_nodes);
+                    }
+                } else {
+                    atomicAdd(&J->nodes, nodes);
+                }
+                have = false;
+                tgt = 0;
+            }
             if (!have) {
                 // ---- fetch a work item: (cube, bound, prefix length, prefix) ----
-                uint32_t plen, item;
+                uint32_t item;
+                bool fresh;
                 if (!QUEUE) {
                     unsigned long long mask = __ballot(1);
                     uint32_t rank = __popcll(mask & lanemask_lt());
@@ -709,8 +736,11 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                         done = true;
                         break;
                     }
-                    c = (uint32_t)A.active[w / A.n_prefix];
+                    uint32_t cw = (uint32_t)A.active[w / A.n_prefix];
                     item = (uint32_t)(w % A.n_prefix);
+                    if (cw != c) hold = -1;  // one launch is one round: same cube, same bound
+                    fresh = false;
+                    c = cw;
                     J = &A.jobs[c];
                     if (ld_c(&J->found)) continue;
                     bound = J->bound;
@@ -719,7 +749,7 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                     // next item of this lane's chunk: same cube and round, no atomics
                     --chunk_left;
                     item = ++item_cur;
-                    plen = cur_plen;
+                    fresh = false;
                     if (ld_c(&J->found)) {
                         chunk_left = 0;
                         end_chunk(A, c, J, chunk_nodes);
@@ -771,8 +801,8 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                         break;
                     }
                     idle_since = 0;
-                    plen = t_plen(old);
-                    const uint32_t n_items = A.pre_n[plen];
+                    const uint32_t pl = t_plen(old);
+                    const uint32_t n_items = A.pre_n[pl];
                     item = t_next(old) + (rank << clog);
                     bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < n_items;
                     unsigned long long inval = __ballot(!valid);
@@ -781,8 +811,10 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                     c = cc;
                     J = &A.jobs[c];
                     bound = t_bound(old);
-                    cur_plen = plen;
+                    plen = pl;
                     item_cur = item;
+                    fresh = true;  // a chunk may follow a reuse of the slot: start from the root
+                    hold = -1;
                     chunk_left = min(item + (1u << clog), n_items) - item - 1u;
                     chunk_nodes = 0;
                     if (ld_c(&J->found)) {
@@ -791,69 +823,63 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                         continue;
                     }
                 }
-                // ---- replay the prefix with all six frames (kernels.cu:1552-1614) ----
-                const uint8_t* pre = (QUEUE ? A.pre_tab[plen] : A.prefixes) + (unsigned long long)item * plen;
-                hp = 0;
-                if (QUEUE && A.serving) {  // the slot may have held another cube: read past the L2
-                    static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame_c(&J->frame[j]); });
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) hp |= ld_c(&J->h[j]) << (5 * j);
-                } else {
-                    static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame(&J->frame[j]); });
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
-                }
-                nodes = 0;
-                g = 0;
-                last = -1;
-                bool ok = true;
-                for (; g < (int)plen && ok; ++g) {
-                    int pm = pre[g];
-                    ++nodes;
-                    uint32_t axis = (uint32_t)((pm / 3) % 3);
-                    if ((excl_axes(hp, bound - (uint32_t)g) >> axis) & 1u) {
-                        ok = false;
-                        break;
-                    }
-                    static_for<0, 6>([&](auto j) {
-                        Frame& fj = f.template at<j>();
-                        frame_do(L, fj, j, pm);
-                        if ((uint32_t)j != 3u + axis) {  // the inherited inverse frame keeps its coset
-                            uint32_t v = frame_lb<K, F>(L, classidx, table, fj);
-                            uint32_t hj = hget(hp, j);
-                            hp = hset(hp, j, (j < 3 && hj) ? max(v, hj - 1u) : v);
-                        }
-                    });
-                    probes += 5;
-#pragma unroll
-                    for (int j = 0; j < 6; ++j)
-                        if ((uint32_t)g + 1u + hget(hp, j) > bound) ok = false;
-                    if (ok && all_tight(hp, bound - (uint32_t)g - 1u)) ok = false;
-                    pth[g * nt] = (uint8_t)pm;
-                    last = pm;
-                }
-                if (!ok) {
-                    all_nodes += nodes;
-                    if (QUEUE) {
-                        chunk_nodes += nodes;
-                        if (!chunk_left) end_chunk(A, c, J, chunk_nodes);
-                    } else {
-                        atomicAdd(&J->nodes, nodes);
-                    }
+                // ---- start the item: its prefix is replayed move by move through
+                // the probe step below, all six frames (kernels.cu:1552-1614) ----
+                const unsigned long long nb = (QUEUE ? A.pre_tab[plen] : A.prefixes)[item];
+                fresh = fresh || hold < 0;
+                unsigned long long x = nb ^ pbits;
+                int shared = fresh ? 0 : min((int)plen, x ? (int)(__builtin_ctzll(x) >> 3) : 8);
+                pbits = nb;
+                have = true;
+                if (!fresh && hfail >= 0 && shared > hfail) {
+                    // the previous item's failing move is on this prefix too: same
+                    // nodes as a full replay (moves up to the failing one), no probes
+                    nodes = (unsigned long long)hfail + 1u;
+                    tgt = -1;
                     continue;
                 }
-                base = g;
-                if ((uint32_t)g == bound) {
-                    if (is_solved(f.f0)) report_solution(A, J, QUEUE, g, pth, nt);
-                    todo = 0;
-                } else {
-                    todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
-                    ord = frame_order(hp, bound - (uint32_t)g);
+                hfail = -1;
+                if (fresh) {
+                    hp = 0;
+                    if (QUEUE && A.serving) {  // the slot may have held another cube: read past the L2
+                        static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame_c(&J->frame[j]); });
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) hp |= ld_c(&J->h[j]) << (5 * j);
+                    } else {
+                        static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame(&J->frame[j]); });
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
+                    }
+                    g = 0;
+                    last = -1;
+                    ord = frame_order(hp, bound);
                 }
-                have = true;
+                // moves shared with the held state count as replayed (node counts are
+                // those of a replay from the root)
+                nodes = (unsigned long long)shared;
+                tgt = shared;  // undo down to the shared depth first
+                todo = 0;
                 continue;
             }
-            if (todo && !(((++iter & A.poll_mask) == 0) && ld_c(&J->found))) {
+            if ((uint32_t)g < plen && g <= tgt) {
+                // ---- next prefix move, evaluated like a child ----
+                tgt = (int)plen;
+                m = (int)((pbits >> (8 * g)) & 0xFFu);
+                ++nodes;
+                uint32_t axis = (uint32_t)((m / 3) % 3);
+                if ((excl_axes(hp, bound - (uint32_t)g) >> axis) & 1u) {  // the move is not applied
+                    hfail = g;
+                    m = -1;
+                    tgt = -1;
+                    continue;
+                }
+                cp = hp;
+                rem = ord & ~(0x41041u << (3 + axis));
+                k = next_frame(rem);
+                eval = true;
+                break;
+            }
+            if (g >= (int)plen && todo && !(((++iter & A.poll_mask) == 0) && ld_c(&J->found))) {
                 // ---- pick the next canonical child ----
                 m = (int)ctz(todo);
                 todo &= todo - 1;
@@ -865,25 +891,16 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
                 eval = true;
                 break;
             }
-            // ---- node exhausted: pop, or finish the item ----
-            if (g == base || ld_c(&J->found)) {
-                all_nodes += nodes;
-                if (QUEUE) {
-                    chunk_nodes += nodes;
-                    if (!chunk_left || ld_c(&J->found)) {
-                        chunk_left = 0;
-                        end_chunk(A, c, J, chunk_nodes);
-                    }
-                } else {
-                    atomicAdd(&J->nodes, nodes);
-                }
-                have = false;
+            // ---- node exhausted: finish the item, or pop (also undoing to a shared prefix) ----
+            // (here g <= tgt means g == plen: the item's search is complete)
+            if (g <= tgt || (tgt == (int)plen && ld_c(&J->found))) {
+                tgt = -1;
                 continue;
             }
             --g;
             int pm = pth[g * nt];
             static_for<0, 6>([&](auto j) { frame_undo(L, f.template at<j>(), j, pm); });
-            hp = hst[(g - base) * nt];
+            hp = hst[g * nt];
             last = g > 0 ? (int)pth[(g - 1) * nt] : -1;
             todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g)) & ~((2u << pm) - 1u);
             ord = frame_order(hp, bound - (uint32_t)g);
@@ -897,9 +914,8 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
             // ---- one probe: frame k of child m ----
             uint32_t parent = hget(hp, k);
             Frame t = pick_frame(f, k);
-            int mk = L.frame_move[k < 3 ? k : k - 3][m];
-            if (k < 3) apply_move(L, t, mk);
-            else apply_left(L, t, inverse_move(mk));
+            if (k < 3) right_mul(t, L.rr[0][k][m]);
+            else left_mul(t, L.lr[0][k - 3][m]);
             uint32_t v = frame_lb<K, F>(L, classidx, table, t);
             ++probes;
             if (k < 3 && parent) v = max(v, parent - 1u);
@@ -917,6 +933,48 @@ __global__ void __launch_bounds__(SEARCH_WG) search_kernel(SearchArgs A) {
             atomicAdd(&A.stats[1], all_nodes);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Gather micro-benchmark: random 32-byte table blocks read the way a probe
+// reads them (block minimum word + one offset word), to measure the random-
+// probe ceiling of the memory system; mode 2 streams the table for a
+// calibration of the fetch counters.  Four independent probes per thread per
+// iteration keep several misses in flight.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) gather_kernel(const uint32_t* __restrict__ blk, unsigned long long n_blocks,
+                                                     unsigned long long n_probes, unsigned long long* sum) {
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    unsigned long long tid = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+    uint32_t acc = 0;
+    if (MODE == 2) {  // stream: every word once
+        const uint4* p = reinterpret_cast<const uint4*>(blk);
+        unsigned long long n4 = n_blocks * 2;
+        for (unsigned long long i = tid; i < n4; i += stride) {
+            uint4 v = p[i];
+            acc += v.x ^ v.y ^ v.z ^ v.w;
+        }
+    } else {
+        for (unsigned long long i = tid * 4; i < n_probes; i += stride * 4) {
+            uint32_t v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                unsigned long long h = mix64(i + k + 0x9E3779B97F4A7C15ull);
+                const uint32_t* b = blk + (h % n_blocks) * 8u;
+                uint32_t w = (uint32_t)(h >> 61);
+                if (MODE == 1) v[k] = __builtin_nontemporal_load(b) + __builtin_nontemporal_load(b + w);
+                else v[k] = b[0] + b[w];
+            }
+            acc += v[0] ^ v[1] ^ v[2] ^ v[3];
+        }
+    }
+    atomicAdd(sum, (unsigned long long)acc);
 }
 
 template <int K>
@@ -937,7 +995,7 @@ void* search_fn(int F, bool queue) {
 
 }  // namespace
 
-size_t search_smem_bytes(int block) { return sizeof(CubeLUT) + (size_t)block * (MAX_REL * 4 + MAX_BOUND); }
+size_t search_smem_bytes(int block) { return sizeof(CubeLUT) + (size_t)block * (STACK_DEPTH * 4 + STACK_DEPTH); }
 
 hipError_t search_occupancy(int K, int F, bool queue, int block, int* blocks_per_cu) {
     void* fn = nullptr;
@@ -1011,6 +1069,14 @@ hipError_t launch_bfs_layer(int K, bool forward, int grid, const CubeLUT* lut, u
 hipError_t launch_histogram(int grid, const uint32_t* tbl, unsigned long long n_words, unsigned long long n_entries,
                             unsigned long long* counts, hipStream_t s) {
     hipLaunchKernelGGL(histogram_kernel, dim3(grid), dim3(256), 0, s, tbl, n_words, n_entries, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(int mode, int grid, const uint32_t* blk, unsigned long long n_blocks,
+                         unsigned long long n_probes, unsigned long long* sum, hipStream_t s) {
+    if (mode == 0) hipLaunchKernelGGL(gather_kernel<0>, dim3(grid), dim3(256), 0, s, blk, n_blocks, n_probes, sum);
+    else if (mode == 1) hipLaunchKernelGGL(gather_kernel<1>, dim3(grid), dim3(256), 0, s, blk, n_blocks, n_probes, sum);
+    else hipLaunchKernelGGL(gather_kernel<2>, dim3(grid), dim3(256), 0, s, blk, n_blocks, n_probes, sum);
     return hipGetLastError();
 }
 

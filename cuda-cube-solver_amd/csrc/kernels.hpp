@@ -9,6 +9,10 @@
 #ifndef SEARCH_WG
 #define SEARCH_WG 256
 #endif
+// Waves per SIMD the search kernel is compiled for (caps it at 512 / N VGPRs).
+#ifndef SEARCH_WAVES_PER_SIMD
+#define SEARCH_WAVES_PER_SIMD 3
+#endif
 
 namespace ccs {
 
@@ -31,5 +35,9 @@ hipError_t launch_histogram(int grid, const uint32_t* tbl, unsigned long long n_
 // block table (blk) for the reference's saturated-offset values.
 hipError_t launch_encode(int format, const uint32_t* tbl, const uint32_t* blk, unsigned long long n_entries,
                          unsigned long long b0, unsigned long long nb, uint32_t* out, unsigned int* bad, hipStream_t s);
+
+// mode 0: random probes; 1: random probes with nontemporal loads; 2: stream the table once.
+hipError_t launch_gather(int mode, int grid, const uint32_t* blk, unsigned long long n_blocks,
+                         unsigned long long n_probes, unsigned long long* sum, hipStream_t s);
 
 }  // namespace ccs

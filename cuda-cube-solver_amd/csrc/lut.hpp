@@ -17,16 +17,29 @@ namespace ccs {
 
 constexpr uint32_t SEL_ZERO = 0x0Cu;
 
+// Move records.  A forward frame j (the cube seen in axis frame j) takes a
+// move as right multiplication by the move as that frame sees it; the inverse
+// frame 3 + j takes it as left multiplication by that move's inverse.
+struct alignas(16) RightRec {
+    uint32_t csel[2];   // corner word w: source corner index per byte (perm over c1:c0)
+    uint32_t cadd[2];   // co << 3 added per byte
+    uint32_t elo[3];    // edge word w: source index < 8 (perm over e1:e0), else SEL_ZERO
+    uint32_t ehi[3];    // source index - 8 for sources >= 8 (perm over e2:e2), else SEL_ZERO
+    uint32_t eflip[3];  // eo << 4 xored per byte
+    uint32_t pad[3];
+};
+struct alignas(16) LeftRec {
+    uint32_t lc[2];     // corners: move.cp[c] | move.co[c] << 3 for c = 0..7
+    uint32_t le[3];     // edges:   move.ep[c] | move.eo[c] << 4 for c = 0..11
+    uint32_t pad[3];
+};
+
 struct alignas(16) CubeLUT {
-    // right multiplication by HTM move m (state then move)
-    uint32_t mc_sel[18][2];   // corner word w: source corner index per byte (perm over c1:c0)
-    uint32_t mc_add[18][2];   // co << 3 added per byte
-    uint32_t me_lo[18][3];    // edge word w: source index < 8 (perm over e1:e0), else SEL_ZERO
-    uint32_t me_hi[18][3];    // source index - 8 for sources >= 8 (perm over e2:e2), else SEL_ZERO
-    uint32_t me_flip[18][3];  // eo << 4 xored per byte
-    // left multiplication (move then state): byte of the move for each cubie
-    uint32_t lc[18][2];       // corners: move.cp[c] | move.co[c] << 3 for c = 0..7
-    uint32_t le[18][3];       // edges:   move.ep[c] | move.eo[c] << 4 for c = 0..11
+    // [0][j][m] applies move m to frame j (forward) or 3 + j (inverse), [1][j][m]
+    // undoes it; [0][0][m] is plain right multiplication by m, [1][0][m] plain
+    // left multiplication by m.
+    RightRec rr[2][3][18];
+    LeftRec lr[2][3][18];
     // edge conjugation by UD symmetry s (M g M^-1): position part, then cubie relabel
     uint32_t cs_lo[16][3];
     uint32_t cs_hi[16][3];
@@ -43,8 +56,7 @@ struct alignas(16) CubeLUT {
     uint16_t slice_unrank[496];
     uint8_t eorder_unrank[24];
     uint16_t gen_rep[2][48];  // class -> E content, by flip parity of the non-E edges
-    uint8_t frame_move[3][18];// move m seen in axis frame k
-    uint8_t pad0[2];
+    uint16_t b3digits[256];   // base-3 byte (< 243) -> its five digits, two bits each
     uint32_t amask[19];       // canonical successors of the last move (index last + 1), 18-bit mask
     uint32_t pad1;
 };

@@ -7,7 +7,9 @@
 namespace ccs {
 
 constexpr int MAX_BOUND = 24;   // longest searched sequence (the reference's MAX_DEPTH, kernels.cu:700)
-constexpr int MAX_REL = 20;     // stacked DFS levels below a prefix of length >= 4
+// Depth of the per-lane search stack.  Every legal position is within 20 face
+// turns, so no search of a legal cube goes deeper; bounds above it are refused.
+constexpr int STACK_DEPTH = 20;
 constexpr int MAX_PLEN = 7;     // longest work-item prefix
 
 // One cube being searched, resident in HBM (the role of the reference's
@@ -74,14 +76,14 @@ struct SearchArgs {
     const int* active;
     int n_active;
     int plen;
-    const uint8_t* prefixes;
+    const unsigned long long* prefixes;  // one item per word, move j in byte j
     unsigned long long n_prefix;
     unsigned long long* counter;
     // persistent form
     QueueCtl* q;
     unsigned long long* ring;
     unsigned long long ring_size;
-    const uint8_t* pre_tab[MAX_PLEN + 1];
+    const unsigned long long* pre_tab[MAX_PLEN + 1];
     unsigned int pre_n[MAX_PLEN + 1];
     int prefix_len;
     int max_bound;

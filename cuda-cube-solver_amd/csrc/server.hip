@@ -211,7 +211,7 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
         A.pre_n[p] = t->prefix_n[p] ? t->prefix_n[p] : 0xFFFFFFFFu;
     }
     A.prefix_len = o.prefix_len;
-    A.max_bound = MAX_BOUND;
+    A.max_bound = STACK_DEPTH;
     A.deeper0 = (unsigned long long)o.deeper_at[0];
     A.deeper1 = (unsigned long long)o.deeper_at[1];
     A.poll_mask = 1023;

@@ -158,7 +158,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     A.table = t->d_table[t->format];
     A.jobs = d_jobs;
     A.prefix_len = o.prefix_len;
-    A.max_bound = o.max_bound;
+    A.max_bound = std::min(o.max_bound, STACK_DEPTH);
     A.deeper0 = d0;
     A.deeper1 = d1;
     A.poll_mask = (unsigned)o.poll_interval - 1u;
@@ -185,7 +185,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             std::map<int, std::vector<int>> groups;
             for (int i = 0; i < n; ++i) {
                 if (finished[i]) continue;
-                if ((int)jobs[i].bound > o.max_bound) {
+                if ((int)jobs[i].bound > std::min(o.max_bound, STACK_DEPTH)) {
                     finished[i] = 1;
                     continue;
                 }

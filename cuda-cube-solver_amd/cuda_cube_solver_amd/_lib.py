@@ -97,6 +97,7 @@ SIGNATURES = {
     "ccs_server_submit": (I, [P, P, U64]),
     "ccs_server_poll": (I, [P, P, I, I]),
     "ccs_server_stop": (I, [P]),
+    "ccs_gather_bench": (I, [P, I, U64, I, P, P]),
     "ccs_index_host": (I, [CP, P, I, P]),
 }
 

@@ -150,7 +150,7 @@ class GpuSolver:
 
     def __init__(self, tables: Path = Path("tables"), kind: str = "d4t", table_format: str = "block",
                  mode: str = "plain", prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0,
-                 poll_interval: int = 1024, table: Tables | None = None):
+                 poll_interval: int = 1024, chunk_nodes: int = 0, table: Tables | None = None):
         if table is None:
             tables = Path(tables)
             if tables.suffix == ".bin":
@@ -164,6 +164,7 @@ class GpuSolver:
         self.deeper_prefix_at = tuple(deeper_prefix_at)
         self.blocks_per_cu = blocks_per_cu
         self.poll_interval = poll_interval
+        self.chunk_nodes = chunk_nodes
         self.last_stats: _lib.SolveStats | None = None
 
     def _opts(self, mode: str | None = None) -> _lib.SolveOpts:
@@ -175,6 +176,7 @@ class GpuSolver:
         o.blocks_per_cu = self.blocks_per_cu
         o.max_bound = 0
         o.poll_interval = self.poll_interval
+        o.chunk_nodes = self.chunk_nodes
         return o
 
     def solve(self, states: list[CubieState], log=None, mode: str | None = None) -> list[Solution]:

@@ -190,6 +190,13 @@ int ccs_server_stop(ccs_server_t* s);
 
 /* ---- diagnostics ------------------------------------------------------- */
 
+/* Random-probe micro-benchmark on the resident solving table: n_probes
+ * random 32-byte blocks read as a probe reads them (mode 0; mode 1 with
+ * nontemporal loads), or one streaming pass over the table (mode 2).  The
+ * measured rate is the memory system's ceiling for the search's probes.
+ * Counterpart of the reference's bench/gpu_gather.py. */
+int ccs_gather_bench(ccs_tables_t* t, int mode, uint64_t n_probes, int blocks_per_cu, double* ms, uint64_t* checksum);
+
 /* Table index of n cubes computed on the host with the same LDS lookup-table
  * arithmetic the kernels use (no GPU; for checking that arithmetic). */
 int ccs_index_host(const char* kind, const uint8_t* cubes, int n, uint64_t* idx_out);
