@@ -19,7 +19,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "cuda-cube-solver_amd"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_PROBE = 32  # one 32-byte table block per lookup
+BYTES_PER_PROBE = 32  # algorithmic: one 32-byte table block per lookup
+# HBM bytes moved per probe by the search kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch) scaled by a 256 MiB copy kernel's readings
+# (FETCH_SIZE x2, WRITE_SIZE x1): profiles/search_kernel_d4t_pmc.md.  A random
+# 32-byte block costs one 128-byte line.
+FETCHED_BYTES_PER_PROBE = 128.6
+WRITTEN_BYTES_PER_PROBE = 0.04
 
 
 def log(msg: str) -> None:
@@ -32,6 +38,27 @@ def pct(xs, q):
         return None
     k = max(0, min(len(xs) - 1, int(round(q / 100.0 * len(xs) + 0.5)) - 1))  # nearest rank
     return xs[k]
+
+
+def gather_ceiling(tb) -> dict:
+    """Random-probe ceiling of the memory system on this table: 2^32 reads of a
+    random 32-byte block's minimum word plus one offset word (the search's
+    access pattern, four independent probes per lane), and a streaming read of
+    the whole table, timed with HIP events in the library."""
+    import ctypes
+
+    from cuda_cube_solver_amd import _lib
+
+    out = {}
+    for mode, n in ((0, 1 << 32), (2, 0)):
+        ms, cs = ctypes.c_double(), ctypes.c_uint64()
+        _lib.check(_lib.lib().ccs_gather_bench(tb.handle, mode, n, 16, ctypes.byref(ms), ctypes.byref(cs)))
+        if mode == 0:
+            out["random_probes_g_per_s"] = round(n / ms.value / 1e6, 2)
+            out["random_probe_algorithmic_gb_s"] = round(n * BYTES_PER_PROBE / ms.value / 1e6, 1)
+        else:
+            out["stream_gb_s"] = round(tb.nbytes("block") / ms.value / 1e6, 1)
+    return out
 
 
 def cpu_baseline(tb, cubes, nodes_per_cube: float, seconds_budget: float):
@@ -65,8 +92,10 @@ def main() -> None:
     ap.add_argument("--plain", type=int, default=128, help="cubes solved one at a time for the plain figure")
     ap.add_argument("--table", default="d4t")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gather", action="store_true", help="skip the random-probe ceiling measurement")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--prefix-len", type=int, default=4)
+    ap.add_argument("--chunk-nodes", type=int, default=0, help="persistent form: nodes per fetched chunk (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver shards by "
@@ -82,7 +111,8 @@ def main() -> None:
     tb, counts = Tables.build(args.table, "block", log=log)
     log(f"built {args.table} in {time.perf_counter() - t:.1f}s, depth counts {counts}")
     cubes = states.uniform_states(args.cubes, args.seed)
-    solver = GpuSolver(table=tb, mode="batched", prefix_len=args.prefix_len, blocks_per_cu=args.blocks_per_cu)
+    solver = GpuSolver(table=tb, mode="batched", prefix_len=args.prefix_len, blocks_per_cu=args.blocks_per_cu,
+                       chunk_nodes=args.chunk_nodes)
 
     for w in range(args.warmup):
         solver.solve(cubes[: min(512, len(cubes))], mode="batched")
@@ -112,9 +142,10 @@ def main() -> None:
         sol = solver.solve([s], mode="plain")[0]
         plain_ms.append((time.perf_counter() - t) * 1e3)
         plain_len.append(len(sol.moves))
-    plain_value = len(plain_ms) / (sum(plain_ms) / 1e3)
+    plain_value = len(plain_ms) / (sum(plain_ms) / 1e3) if plain_ms else None
     same = plain_len == lengths[: len(plain_len)]
-    log(f"plain {len(plain_ms)} cubes: {plain_value:.2f} cubes/s, lengths identical to batched: {same}")
+    if plain_ms:
+        log(f"plain {len(plain_ms)} cubes: {plain_value:.2f} cubes/s, lengths identical to batched: {same}")
 
     achieved = probes * BYTES_PER_PROBE / (kernel_ms / 1e3) / 1e9
     line = {
@@ -135,17 +166,20 @@ def main() -> None:
                    "prefix_len": args.prefix_len},
         "batched_p50_ms": round(pct(lat, 50), 2),
         "batched_p99_ms": round(pct(lat, 99), 2),
-        "plain_cubes_per_s": round(plain_value, 3),
-        "plain_p50_ms": round(pct(plain_ms, 50), 2),
-        "plain_p99_ms": round(pct(plain_ms, 99), 2),
+        "plain_cubes_per_s": round(plain_value, 3) if plain_ms else None,
+        "plain_p50_ms": round(pct(plain_ms, 50), 2) if plain_ms else None,
+        "plain_p99_ms": round(pct(plain_ms, 99), 2) if plain_ms else None,
         "plain_cubes": len(plain_ms),
-        "batched_over_plain": round(value / plain_value, 2),
+        "batched_over_plain": round(value / plain_value, 2) if plain_ms else None,
         "lengths_identical": same,
         "length_histogram": {str(L): lengths.count(L) for L in sorted(set(lengths))},
         "probes_per_cube": round(probes / (args.cubes * args.steps)),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(probes / args.steps * (FETCHED_BYTES_PER_PROBE + WRITTEN_BYTES_PER_PROBE))},
     }
+    if not args.no_gather:
+        line["probe_ceiling"] = gather_ceiling(tb)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(tb, cubes, nodes_total / (args.cubes * args.steps), 20.0)
     print(json.dumps(line), flush=True)

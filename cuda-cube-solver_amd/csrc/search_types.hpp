@@ -10,6 +10,9 @@ constexpr int MAX_BOUND = 24;   // longest searched sequence (the reference's MA
 // Depth of the per-lane search stack.  Every legal position is within 20 face
 // turns, so no search of a legal cube goes deeper; bounds above it are refused.
 constexpr int STACK_DEPTH = 20;
+// Persistent form: nodes a lane aims to take per fetch (a 1,024-cube d4t sweep
+// of 16K..1M peaks at 256K; at least 4,096 chunks per round keep one cube wide).
+constexpr unsigned long long DEFAULT_CHUNK_NODES = 262144;
 constexpr int MAX_PLEN = 7;     // longest work-item prefix
 
 // One cube being searched, resident in HBM (the role of the reference's

@@ -162,7 +162,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     A.deeper0 = d0;
     A.deeper1 = d1;
     A.poll_mask = (unsigned)o.poll_interval - 1u;
-    A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : 4096ull;
+    A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : DEFAULT_CHUNK_NODES;
     A.stats = d_stats;
     for (int p = 0; p <= MAX_PLEN; ++p) {
         A.pre_tab[p] = t->d_prefix[p];
