@@ -202,19 +202,23 @@ CCS_HD uint32_t table_lb(const uint8_t* __restrict__ tbl, uint64_t idx) {
 // Lower bound of a frame from the solving table, with the block number and
 // the entry within the block computed in 32-bit arithmetic: with
 // N_EDGE = Q * D + R, index / D = class * Q + (class * R + edge) / D, and the
-// bracket stays below 2^23 (class * N_EDGE itself reaches 2^35).
+// bracket stays below 2^23 (class * N_EDGE itself reaches 2^35).  Split in two
+// so that a caller can put several probes' table reads in flight before it
+// uses any: probe_block (from the frame's classidx word) and block_value.
 template <int K, int F>
-CCS_HD uint32_t frame_bound(const CubeLUT& L, const uint32_t* __restrict__ classidx,
-                            const uint8_t* __restrict__ tbl, const Frame& f) {
+CCS_HD const uint8_t* probe_block(const CubeLUT& L, uint32_t packed, const uint8_t* __restrict__ tbl,
+                                  const Frame& f, uint32_t& j) {
     constexpr uint32_t D = F == 0 ? 124u : 155u;
     constexpr uint32_t Q = (uint32_t)(KindT<K>::N_EDGE / D), R = (uint32_t)(KindT<K>::N_EDGE % D);
     static_assert(Q < (1u << 24), "class * Q must stay a 24-bit product");
-    const uint32_t packed = classidx[corner_raw(L, f)];
     const uint32_t cls = packed >> 4;
     const uint32_t x = mul24(cls, R) + edge_coord<K>(L, f, packed & 15u);
     const uint32_t q = x / D;
-    const uint32_t j = x - mul24(q, D);
-    const uint8_t* p = tbl + (uint64_t)(mul24(cls, Q) + q) * 32u;
+    j = x - mul24(q, D);
+    return tbl + (uint64_t)(mul24(cls, Q) + q) * 32u;
+}
+template <int F>
+CCS_HD uint32_t block_value(const CubeLUT& L, const uint8_t* p, uint32_t j) {
     if constexpr (F == 0) {
         const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
         const uint32_t pos = 8u + 2u * j;
@@ -225,6 +229,13 @@ CCS_HD uint32_t frame_bound(const CubeLUT& L, const uint32_t* __restrict__ class
         const uint32_t m = hdr & 63u;
         return d == 0u ? m : m + (hdr >> 6) + d;
     }
+}
+template <int K, int F>
+CCS_HD uint32_t frame_bound(const CubeLUT& L, const uint32_t* __restrict__ classidx,
+                            const uint8_t* __restrict__ tbl, const Frame& f) {
+    uint32_t j;
+    const uint8_t* p = probe_block<K, F>(L, classidx[corner_raw(L, f)], tbl, f, j);
+    return block_value<F>(L, p, j);
 }
 
 // Representative frame of a table entry (the BFS expands it); any member of

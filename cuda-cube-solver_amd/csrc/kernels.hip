@@ -347,23 +347,26 @@ __device__ __forceinline__ bool all_tight(uint32_t cp, uint32_t budget) {
     uint32_t b3 = budget | (budget << 5) | (budget << 10);
     return (cp & 0x7FFFu) == b3 || ((cp >> 15) & 0x7FFFu) == b3;
 }
-// Frame probe order: frames at the budget edge, then one below, then the rest
-// (kernels.cu:1398-1410).
+// Frame probe order (kernels.cu:1398-1410 orders all six frames: at the budget
+// edge, one below, the rest).  Here a step probes one forward and one inverse
+// frame together, so the order is kept per group: forward frame j at bit
+// 4 * class + j, inverse frame 3 + j at bit 16 + 4 * class + j (class 0 = at
+// the edge, 1 = one below, 2 = the rest); the lowest set bit goes first.  The
+// order decides only which probes a pruned child costs, never the nodes.
 __device__ __forceinline__ uint32_t frame_order(uint32_t hp, uint32_t edge) {
-    uint32_t at = 0, below = 0;
+    uint32_t o = 0;
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
         uint32_t h = hget(hp, j);
-        at |= (uint32_t)(h >= edge) << j;
-        below |= (uint32_t)(h + 1u == edge) << j;
+        uint32_t sh = h >= edge ? 0u : (h + 1u == edge ? 4u : 8u);
+        o |= 1u << (sh + (uint32_t)(j % 3) + (j < 3 ? 0u : 16u));
     }
-    return at | (below << 6) | ((~(at | below) & 0x3Fu) << 12);
+    return o;
 }
-__device__ __forceinline__ int next_frame(uint32_t& rem) {
-    if (!rem) return 6;
-    uint32_t b = ctz(rem);
-    rem &= rem - 1;
-    return (int)(b % 6u);
+// Axis (U/D = 0, R/L = 1, F/B = 2) of HTM move m < 18: face m / 3, mod 3.
+__device__ __forceinline__ uint32_t move_axis(int m) {
+    uint32_t face = ((uint32_t)m * 11u) >> 5;
+    return face >= 3u ? face - 3u : face;
 }
 // The six frames of a lane as named registers (an array indexed in loops can
 // be left in scratch memory by the compiler).
@@ -386,24 +389,11 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
         static_for<I + 1, N>(fn);
     }
 }
-__device__ __forceinline__ uint32_t sel6(int k, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t e,
-                                         uint32_t g) {
-    uint32_t t = a;
-    t = k == 1 ? b : t;
-    t = k == 2 ? c : t;
-    t = k == 3 ? d : t;
-    t = k == 4 ? e : t;
-    return k == 5 ? g : t;
-}
-// Frame k, selected word by word (whole-struct selects become memory copies).
-__device__ __forceinline__ Frame pick_frame(const Frames6& f, int k) {
-    Frame t;
-    t.c0 = sel6(k, f.f0.c0, f.f1.c0, f.f2.c0, f.f3.c0, f.f4.c0, f.f5.c0);
-    t.c1 = sel6(k, f.f0.c1, f.f1.c1, f.f2.c1, f.f3.c1, f.f4.c1, f.f5.c1);
-    t.e0 = sel6(k, f.f0.e0, f.f1.e0, f.f2.e0, f.f3.e0, f.f4.e0, f.f5.e0);
-    t.e1 = sel6(k, f.f0.e1, f.f1.e1, f.f2.e1, f.f3.e1, f.f4.e1, f.f5.e1);
-    t.e2 = sel6(k, f.f0.e2, f.f1.e2, f.f2.e2, f.f3.e2, f.f4.e2, f.f5.e2);
-    return t;
+// Frame k of three, selected word by word (whole-struct selects become memory copies).
+__device__ __forceinline__ Frame pick3(const Frame& a, const Frame& b, const Frame& c, uint32_t k) {
+    return Frame{k == 0u ? a.c0 : (k == 1u ? b.c0 : c.c0), k == 0u ? a.c1 : (k == 1u ? b.c1 : c.c1),
+                 k == 0u ? a.e0 : (k == 1u ? b.e0 : c.e0), k == 0u ? a.e1 : (k == 1u ? b.e1 : c.e1),
+                 k == 0u ? a.e2 : (k == 1u ? b.e2 : c.e2)};
 }
 __device__ __forceinline__ Frame load_frame(const Frame* p) {
     const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
@@ -665,8 +655,15 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     uint32_t hp = 0, cp = 0;
     uint32_t c = 0xFFFFFFFFu;  // cube of the current item
     SearchJob* J = nullptr;
-    uint32_t bound = 0, todo = 0, ord = 0, rem = 0, iter = 0, idle = 0, plen = 0;
-    int g = 0, tgt = 0, last = -1, m = -1, k = 0;
+    uint32_t bound = 0, todo = 0, ord = 0, iter = 0, idle = 0, plen = 0;
+    int g = 0, tgt = 0, m = -1;
+    // Child m under evaluation: forward / inverse frames still to probe (bits
+    // in frame_order's layout), whether a probe pruned it, whether it is done.
+    uint32_t remF = 0, remI = 0;
+    bool pruned = false, cdone = false;
+    // Phase of the lane's item: 0 fetch one, 1 undo down to the prefix shared
+    // with the previous item, 2 replay the prefix, 3 search below it, 4 done.
+    int phase = 0;
     unsigned long long nodes = 0, probes = 0, all_nodes = 0, idle_since = 0, chunk_nodes = 0;
     unsigned long long pbits = 0;  // the item's prefix, move j in byte j
     uint32_t chunk_left = 0, item_cur = 0;  // persistent form: rest of this lane's chunk
@@ -675,22 +672,25 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // move (-1: none).  The next item of the same cube and round undoes back
     // to the prefix it shares and replays only the rest.
     int hold = -1, hfail = -1;
-    bool have = false, done = false;
+    bool done = false;
 
     for (;;) {
         bool eval = false;
         while (!done) {
+            // One pass resolves the lane's evaluated child, then finishes, fetches,
+            // undoes or backs up as needed and picks the next child, so that the
+            // common sequences (push then pick, discard then pick, pop then pick)
+            // cost the wave a single pass.
             if (m >= 0) {
-                if (k < 6) {  // still evaluating child m
+                if (!cdone) {  // frames of child m still to probe
                     eval = true;
                     break;
                 }
-                if (k == 6 && !all_tight(cp, bound - (uint32_t)g - 1u)) {  // survived: push
+                if (!pruned && !all_tight(cp, bound - (uint32_t)g - 1u)) {  // survived: push
                     static_for<0, 6>([&](auto j) { frame_do(L, f.template at<j>(), j, m); });
                     hst[g * nt] = hp;
                     pth[g * nt] = (uint8_t)m;
                     hp = cp;
-                    last = m;
                     ++g;
                     if ((uint32_t)g == bound) {
                         if (is_solved(f.f0)) report_solution(A, J, QUEUE, g, pth, nt);
@@ -699,14 +699,19 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         todo = L.amask[m + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
                         ord = frame_order(hp, bound - (uint32_t)g);
                     }
-                } else if ((uint32_t)g < plen) {  // a prefix move was pruned: the item ends
+                    if ((uint32_t)g == plen) phase = 3;  // the prefix is in place
+                } else if (phase == 2) {  // a prefix move was pruned: the item ends
                     hfail = g;
-                    tgt = -1;
+                    phase = 4;
                 }
                 m = -1;
-                continue;
             }
-            if (tgt < 0) {
+            bool pop = phase == 1;
+            if (phase == 3 && todo == 0) {  // node exhausted: the item is complete, or back up
+                if ((uint32_t)g == plen || ld_c(&J->found)) phase = 4;
+                else pop = true;
+            }
+            if (phase == 4) {
                 // ---- the item is finished: account its nodes ----
                 hold = (uint32_t)g <= plen ? g : -1;  // deeper only when found ended the search
                 all_nodes += nodes;
@@ -719,10 +724,9 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 } else {
                     atomicAdd(&J->nodes, nodes);
                 }
-                have = false;
-                tgt = 0;
+                phase = 0;
             }
-            if (!have) {
+            if (phase == 0) {
                 // ---- fetch a work item: (cube, bound, prefix length, prefix) ----
                 uint32_t item;
                 bool fresh;
@@ -830,12 +834,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 unsigned long long x = nb ^ pbits;
                 int shared = fresh ? 0 : min((int)plen, x ? (int)(__builtin_ctzll(x) >> 3) : 8);
                 pbits = nb;
-                have = true;
                 if (!fresh && hfail >= 0 && shared > hfail) {
                     // the previous item's failing move is on this prefix too: same
                     // nodes as a full replay (moves up to the failing one), no probes
                     nodes = (unsigned long long)hfail + 1u;
-                    tgt = -1;
+                    phase = 4;
                     continue;
                 }
                 hfail = -1;
@@ -851,7 +854,6 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
                     }
                     g = 0;
-                    last = -1;
                     ord = frame_order(hp, bound);
                 }
                 // moves shared with the held state count as replayed (node counts are
@@ -859,51 +861,54 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 nodes = (unsigned long long)shared;
                 tgt = shared;  // undo down to the shared depth first
                 todo = 0;
-                continue;
+                phase = g > tgt ? 1 : ((uint32_t)g < plen ? 2 : 3);
+                if (phase == 3) todo = L.amask[0] & ~axis_mask(excl_axes(hp, bound));  // empty prefix
+                pop = phase == 1;
             }
-            if ((uint32_t)g < plen && g <= tgt) {
+            if (pop) {
+                --g;
+                const int pm = pth[g * nt];
+                static_for<0, 6>([&](auto j) { frame_undo(L, f.template at<j>(), j, pm); });
+                hp = hst[g * nt];
+                const int last = g > 0 ? (int)pth[(g - 1) * nt] : -1;
+                todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g)) & ~((2u << pm) - 1u);
+                ord = frame_order(hp, bound - (uint32_t)g);
+                if (phase == 1 && g == tgt) phase = 2;
+            }
+            if (phase == 2) {
                 // ---- next prefix move, evaluated like a child ----
-                tgt = (int)plen;
                 m = (int)((pbits >> (8 * g)) & 0xFFu);
                 ++nodes;
-                uint32_t axis = (uint32_t)((m / 3) % 3);
+                const uint32_t axis = move_axis(m);
                 if ((excl_axes(hp, bound - (uint32_t)g) >> axis) & 1u) {  // the move is not applied
                     hfail = g;
                     m = -1;
-                    tgt = -1;
+                    phase = 4;
                     continue;
                 }
-                cp = hp;
-                rem = ord & ~(0x41041u << (3 + axis));
-                k = next_frame(rem);
+                cp = hp;  // the inherited inverse frame keeps the parent's value
+                remF = ord & 0x777u;
+                remI = (ord >> 16) & ~(0x111u << axis);
+                pruned = cdone = false;
                 eval = true;
                 break;
             }
-            if (g >= (int)plen && todo && !(((++iter & A.poll_mask) == 0) && ld_c(&J->found))) {
+            if (phase == 3 && todo) {
+                if (((++iter & A.poll_mask) == 0) && ld_c(&J->found)) {
+                    phase = 4;
+                    continue;
+                }
                 // ---- pick the next canonical child ----
                 m = (int)ctz(todo);
                 todo &= todo - 1;
                 ++nodes;
-                int inherit = 3 + (m / 3) % 3;
                 cp = hp;  // the inherited inverse frame keeps the parent's value
-                rem = ord & ~(0x41041u << inherit);
-                k = next_frame(rem);
+                remF = ord & 0x777u;
+                remI = (ord >> 16) & ~(0x111u << move_axis(m));
+                pruned = cdone = false;
                 eval = true;
                 break;
             }
-            // ---- node exhausted: finish the item, or pop (also undoing to a shared prefix) ----
-            // (here g <= tgt means g == plen: the item's search is complete)
-            if (g <= tgt || (tgt == (int)plen && ld_c(&J->found))) {
-                tgt = -1;
-                continue;
-            }
-            --g;
-            int pm = pth[g * nt];
-            static_for<0, 6>([&](auto j) { frame_undo(L, f.template at<j>(), j, pm); });
-            hp = hst[g * nt];
-            last = g > 0 ? (int)pth[(g - 1) * nt] : -1;
-            todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g)) & ~((2u << pm) - 1u);
-            ord = frame_order(hp, bound - (uint32_t)g);
         }
         if (__all(done)) break;
         if (QUEUE && !__any(eval)) {  // the whole wave is waiting for work
@@ -911,16 +916,35 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             continue;
         }
         if (eval) {
-            // ---- one probe: frame k of child m ----
-            uint32_t parent = hget(hp, k);
-            Frame t = pick_frame(f, k);
-            if (k < 3) right_mul(t, L.rr[0][k][m]);
-            else left_mul(t, L.lr[0][k - 3][m]);
-            uint32_t v = frame_lb<K, F>(L, classidx, table, t);
-            ++probes;
-            if (k < 3 && parent) v = max(v, parent - 1u);
-            cp = hset(cp, k, v);
-            k = ((uint32_t)g + 1u + v > bound) ? 7 : next_frame(rem);
+            // ---- one step of child m: its next forward and next inverse frame,
+            // both table reads in flight together ----
+            const bool hasF = remF != 0u, hasI = remI != 0u;
+            const uint32_t kf = ctz(remF | 0x10000u) & 3u, ki = ctz(remI | 0x10000u) & 3u;
+            remF &= remF - 1u;
+            remI &= remI - 1u;
+            Frame tf = pick3(f.f0, f.f1, f.f2, kf), ti = pick3(f.f3, f.f4, f.f5, ki);
+            right_mul(tf, L.rr[0][kf][m]);
+            left_mul(ti, L.lr[0][ki][m]);
+            const uint32_t pkF = classidx[corner_raw(L, tf)], pkI = classidx[corner_raw(L, ti)];
+            uint32_t jF, jI;
+            const uint8_t* bF = probe_block<K, F>(L, pkF, table, tf, jF);
+            const uint8_t* bI = probe_block<K, F>(L, pkI, table, ti, jI);
+            uint32_t vf = hasF ? block_value<F>(L, bF, jF) : 0u;
+            const uint32_t vi = hasI ? block_value<F>(L, bI, jI) : 0u;
+            probes += (uint32_t)hasF + (uint32_t)hasI;
+            const uint32_t pf = hget(hp, (int)kf);
+            if (pf) vf = max(vf, pf - 1u);  // forward frames: at least the parent's value - 1
+            bool pr = false;
+            if (hasF) {
+                cp = hset(cp, (int)kf, vf);
+                pr = (uint32_t)g + 1u + vf > bound;
+            }
+            if (hasI) {
+                cp = hset(cp, 3 + (int)ki, vi);
+                pr = pr || (uint32_t)g + 1u + vi > bound;
+            }
+            pruned = pr;
+            cdone = pr || (remF | remI) == 0u;
         }
     }
     if (A.stats) {
