@@ -660,7 +660,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // Child m under evaluation: forward / inverse frames still to probe (bits
     // in frame_order's layout), whether a probe pruned it, whether it is done.
     uint32_t remF = 0, remI = 0;
-    bool pruned = false, cdone = false;
+    bool pruned = false, cdone = false, first = false;
     // Phase of the lane's item: 0 fetch one, 1 undo down to the prefix shared
     // with the previous item, 2 replay the prefix, 3 search below it, 4 done.
     int phase = 0;
@@ -890,6 +890,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 remF = ord & 0x777u;
                 remI = (ord >> 16) & ~(0x111u << axis);
                 pruned = cdone = false;
+                first = true;
                 eval = true;
                 break;
             }
@@ -906,6 +907,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 remF = ord & 0x777u;
                 remI = (ord >> 16) & ~(0x111u << move_axis(m));
                 pruned = cdone = false;
+                first = true;
                 eval = true;
                 break;
             }
@@ -918,10 +920,15 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         if (eval) {
             // ---- one step of child m: its next forward and next inverse frame,
             // both table reads in flight together ----
-            const bool hasF = remF != 0u, hasI = remI != 0u;
+            bool hasF = remF != 0u, hasI = remI != 0u;
+            if (first) {  // a child's first step probes only its top frame, where most are pruned
+                if ((ctz(remF | 0x10000u) >> 2) <= (ctz(remI | 0x10000u) >> 2)) hasI = false;
+                else hasF = false;
+                first = false;
+            }
             const uint32_t kf = ctz(remF | 0x10000u) & 3u, ki = ctz(remI | 0x10000u) & 3u;
-            remF &= remF - 1u;
-            remI &= remI - 1u;
+            if (hasF) remF &= remF - 1u;
+            if (hasI) remI &= remI - 1u;
             Frame tf = pick3(f.f0, f.f1, f.f2, kf), ti = pick3(f.f3, f.f4, f.f5, ki);
             right_mul(tf, L.rr[0][kf][m]);
             left_mul(ti, L.lr[0][ki][m]);
