@@ -431,13 +431,18 @@ __device__ __forceinline__ unsigned long long make_ticket(unsigned long long kee
            ((unsigned long long)plen << T_PLEN_SHIFT) | ((unsigned long long)chunk_log << T_CHUNK_SHIFT);
 }
 // Items per fetch for a round: about chunk_nodes nodes per chunk, judged by
-// the previous round's nodes and the usual growth of ~13x per bound.
+// the previous round's nodes and the usual growth of ~13x per bound, but at
+// least enough chunks for this cube's share of the launch's lanes (and never
+// fewer than 4096), so that the last cubes of a batch still fill the GPU.
 __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned long long last_nodes,
                                                   uint32_t n_items) {
     unsigned long long per_item = (last_nodes * 13u) / (n_items ? n_items : 1u) + 1u;
+    unsigned long long live = A.serving ? ld_c(&A.q->tail) - ld_c(&A.q->head)
+                                        : (unsigned long long)(A.q->n_jobs - ld_c(&A.q->n_final));
+    unsigned long long want = A.n_lanes / (live ? live : 1ull);
+    want = want < 4096ull ? 4096ull : want;
     uint32_t lg = 0;
-    // at least 4096 chunks per round, so one cube alone still spreads over the GPU
-    while (lg < 6 && (per_item << (lg + 1)) <= A.chunk_nodes && (n_items >> (lg + 1)) >= 4096u) ++lg;
+    while (lg < 6 && (per_item << (lg + 1)) <= A.chunk_nodes && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
     return lg;
 }
 

@@ -94,6 +94,7 @@ struct SearchArgs {
     unsigned int poll_mask;      // found flag polled every poll_mask + 1 picks
     unsigned long long idle_limit;  // persistent form: device-clock ticks a lane may wait for work
     unsigned long long chunk_nodes; // persistent form: target nodes per fetched chunk of items
+    unsigned long long n_lanes;     // persistent form: lanes of the launch (grid * workgroup)
     // serving
     int serving;
     const ServeRec* mb;

@@ -163,6 +163,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     A.deeper1 = d1;
     A.poll_mask = (unsigned)o.poll_interval - 1u;
     A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : DEFAULT_CHUNK_NODES;
+    A.n_lanes = (unsigned long long)grid * (unsigned long long)o.block;
     A.stats = d_stats;
     for (int p = 0; p <= MAX_PLEN; ++p) {
         A.pre_tab[p] = t->d_prefix[p];
@@ -262,9 +263,11 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             lj[k] = jobs[live[k]];
             uint32_t b = lj[k].bound;
             uint32_t plen = std::min(std::min(b, (uint32_t)o.prefix_len), (uint32_t)MAX_PLEN);
-            // first round: chunks of up to 64 items, at least 4096 chunks per round
+            // first round: chunks of up to 64 items, and at least max(4096, the
+            // cube's share of the lanes) chunks per round
+            const unsigned long long want = std::max<unsigned long long>(4096ull, A.n_lanes / (unsigned long long)nl);
             unsigned long long clog = 0;
-            while (clog < 6 && (t->prefix_n[plen] >> (clog + 1)) >= 4096u) ++clog;
+            while (clog < 6 && (unsigned long long)(t->prefix_n[plen] >> (clog + 1)) >= want) ++clog;
             lj[k].ticket = ((unsigned long long)b << 49) | ((unsigned long long)plen << 54) | (clog << 59);
         }
         const unsigned long long ring_size = 2ull * nl + 1024;
