@@ -93,13 +93,18 @@ def main() -> None:
     ap.add_argument("--table", default="d4t")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-probe ceiling measurement")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use --chunk-nodes etc.")
+    ap.add_argument("--tune-cubes", type=int, default=512, help="autotuner sample (seeded uniform states, seed 33)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--prefix-len", type=int, default=4)
     ap.add_argument("--chunk-nodes", type=int, default=0, help="persistent form: nodes per fetched chunk (0: library default)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-seconds", type=float, default=15.0, help="arrival window per offered load")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver shards by "
-              f"replicating the table per GPU, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: cubes are independent, so "
+              f"several GPUs would each hold a copy of the table and take their share of the cubes",
+              file=sys.stderr)
         sys.exit(2)
 
     from cuda_cube_solver_amd import _lib, states
@@ -111,8 +116,17 @@ def main() -> None:
     tb, counts = Tables.build(args.table, "block", log=log)
     log(f"built {args.table} in {time.perf_counter() - t:.1f}s, depth counts {counts}")
     cubes = states.uniform_states(args.cubes, args.seed)
-    solver = GpuSolver(table=tb, mode="batched", prefix_len=args.prefix_len, blocks_per_cu=args.blocks_per_cu,
-                       chunk_nodes=args.chunk_nodes)
+    knobs = {"chunk_nodes": args.chunk_nodes, "blocks_per_cu": args.blocks_per_cu, "prefix_len": args.prefix_len,
+             "deeper_prefix_at": (5e7, 3e9)}
+    tuned = None
+    if not args.no_tune:
+        from cuda_cube_solver_amd import tune
+
+        t = time.perf_counter()
+        tuned = tune.tune(tb, states.uniform_states(args.tune_cubes, 33), log=log)
+        knobs = tune.knobs(tuned["best"])
+        log(f"autotuner: {len(tuned['trials'])} settings in {time.perf_counter() - t:.1f}s, best {tuned['best']}")
+    solver = GpuSolver(table=tb, mode="batched", **knobs)
 
     for w in range(args.warmup):
         solver.solve(cubes[: min(512, len(cubes))], mode="batched")
@@ -163,7 +177,9 @@ def main() -> None:
         "data": "synthetic: seeded uniform random cube states (splitmix64), d4t table built by the GPU BFS",
         "config": {"workload": f"{args.cubes} seeded uniform random states (seed {args.seed}) on d4t, one "
                                f"persistent batched launch per step", "table": args.table,
-                   "prefix_len": args.prefix_len},
+                   "knobs": {k: (list(v) if isinstance(v, tuple) else v) for k, v in knobs.items()},
+                   "knobs_from": (f"autotuner, {len(tuned['trials'])} settings on {tuned['sample']} seed-33 states"
+                                  if tuned else "command line")},
         "batched_p50_ms": round(pct(lat, 50), 2),
         "batched_p99_ms": round(pct(lat, 99), 2),
         "plain_cubes_per_s": round(plain_value, 3) if plain_ms else None,
@@ -180,6 +196,17 @@ def main() -> None:
     }
     if not args.no_gather:
         line["probe_ceiling"] = gather_ceiling(tb)
+    if not args.no_serve:
+        from cuda_cube_solver_amd import serve_bench
+
+        hard = [c for c, L in zip(cubes, lengths) if L >= 18]
+        line["serving"] = []
+        for frac in (0.5, 0.8):
+            qps = frac * value
+            r = serve_bench.run(tb, qps, max(16, int(qps * args.serve_seconds)), seed=5, hard_pool=hard, log=log)
+            r.pop("lengths")
+            r["load"] = f"{frac:.0%} of the batched capacity"
+            line["serving"].append(r)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(tb, cubes, nodes_total / (args.cubes * args.steps), 20.0)
     print(json.dumps(line), flush=True)

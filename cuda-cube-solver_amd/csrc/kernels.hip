@@ -418,7 +418,9 @@ constexpr int T_BOUND_SHIFT = 49;
 constexpr int T_PLEN_SHIFT = 54;
 constexpr unsigned long long T_CLOSING = 1ull << 57;
 constexpr unsigned long long T_RETIRED = 1ull << 58;  // serving: the cube's slot is free again
-constexpr int T_CHUNK_SHIFT = 59;                     // log2 of the items a lane takes per fetch (0..6)
+constexpr int T_CHUNK_SHIFT = 59;
+// Serving: a fetching wave admits new cubes while fewer rounds than this are queued.
+constexpr unsigned long long ADMIT_BELOW = 256;                     // log2 of the items a lane takes per fetch (0..6)
 
 __device__ __forceinline__ uint32_t t_next(unsigned long long t) { return (uint32_t)(t & T_NEXT_MASK); }
 __device__ __forceinline__ uint32_t t_inflight(unsigned long long t) { return (uint32_t)((t & T_INF_MASK) >> 28); }
@@ -437,7 +439,8 @@ __device__ __forceinline__ unsigned long long make_ticket(unsigned long long kee
 __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned long long last_nodes,
                                                   uint32_t n_items) {
     unsigned long long per_item = (last_nodes * 13u) / (n_items ? n_items : 1u) + 1u;
-    unsigned long long live = A.serving ? ld_c(&A.q->tail) - ld_c(&A.q->head)
+    // cubes admitted and not yet finished (u32 differences stay right across wraparound)
+    unsigned long long live = A.serving ? (unsigned long long)((uint32_t)ld_c(&A.q->mb_head) - ld_c(&A.q->n_final))
                                         : (unsigned long long)(A.q->n_jobs - ld_c(&A.q->n_final));
     unsigned long long want = A.n_lanes / (live ? live : 1ull);
     want = want < 4096ull ? 4096ull : want;
@@ -677,6 +680,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // move (-1: none).  The next item of the same cube and round undoes back
     // to the prefix it shares and replays only the rest.
     int hold = -1, hfail = -1;
+    uint32_t atick = 0;  // serving: fetches since the mailbox was last checked
     bool done = false;
 
     for (;;) {
@@ -773,10 +777,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     if (rank == 0) {
                         for (int tries = 0; tries < 32; ++tries) {
                             unsigned long long h = ld_c(&A.q->head), t = ld_c(&A.q->tail);
-                            if (h >= t) {
-                                if (A.serving && admit_one<K, F>(A, L)) continue;
-                                break;
-                            }
+                            // serving: take new cubes when the ring is empty, and while
+                            // fewer than ADMIT_BELOW rounds are queued on every 16th fetch
+                            // (the mailbox is in host memory; its reads cross the bus)
+                            if (A.serving && (h >= t || (t - h < ADMIT_BELOW && (++atick & 15u) == 0u)) &&
+                                admit_one<K, F>(A, L))
+                                continue;
+                            if (h >= t) break;
                             unsigned long long e = ld_c(&A.ring[h % A.ring_size]);
                             if ((e >> 32) != (h & 0xFFFFFFFFull)) break;  // entry still being published
                             uint32_t ce = (uint32_t)e;

@@ -13,6 +13,9 @@ constexpr int STACK_DEPTH = 20;
 // Persistent form: nodes a lane aims to take per fetch (a 1,024-cube d4t sweep
 // of 16K..1M peaks at 256K; at least 4,096 chunks per round keep one cube wide).
 constexpr unsigned long long DEFAULT_CHUNK_NODES = 262144;
+// Serving: one lane covers ~40k nodes/s, so small chunks keep any one cube
+// from waiting seconds on a single lane.
+constexpr unsigned long long SERVE_CHUNK_NODES = 16384;
 constexpr int MAX_PLEN = 7;     // longest work-item prefix
 
 // One cube being searched, resident in HBM (the role of the reference's

@@ -66,7 +66,7 @@ class SolveStats(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("state_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int32),
                 ("prefix_len", ctypes.c_int32), ("deeper_at", ctypes.c_double * 2),
-                ("blocks_per_cu", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32), ("chunk_nodes", ctypes.c_int32)]
 
 
 class Served(ctypes.Structure):

@@ -26,13 +26,14 @@ class Server:
     (admission stops above it); ``max_inflight`` caps them further."""
 
     def __init__(self, tables: Tables, state_budget_bytes: int = 64 << 20, max_inflight: int = 0,
-                 prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0):
+                 prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0, chunk_nodes: int = 0):
         o = _lib.ServerOpts()
         o.state_budget_bytes = state_budget_bytes
         o.max_inflight = max_inflight
         o.prefix_len = prefix_len
         o.deeper_at[0], o.deeper_at[1] = deeper_prefix_at
         o.blocks_per_cu = blocks_per_cu
+        o.chunk_nodes = chunk_nodes
         self._h = ctypes.c_void_p()
         self.tables = tables
         _lib.check(_lib.lib().ccs_server_start(tables.handle, ctypes.byref(o), ctypes.byref(self._h)))

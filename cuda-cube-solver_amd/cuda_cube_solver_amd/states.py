@@ -53,6 +53,18 @@ def uniform_states(n: int, seed: int) -> list[CubieState]:
     return out
 
 
+def sample_indices(pool: int, n: int, seed: int) -> list[int]:
+    """n distinct indices of range(pool), drawn with seed (partial Fisher-Yates)."""
+    if n > pool:
+        raise ValueError(f"cannot draw {n} of {pool}")
+    rng = SplitMix64(seed)
+    p = list(range(pool))
+    for i in range(n):
+        j = i + rng.below(pool - i)
+        p[i], p[j] = p[j], p[i]
+    return p[:n]
+
+
 def flip_heavy_states(n: int, seed: int) -> list[CubieState]:
     """Corners and edge permutation solved, a random even-sized subset of the 12
     edges flipped in place; the last state is the superflip (all 12)."""

@@ -1,0 +1,89 @@
+"""Autotuner for the persistent (batched) search form.
+
+Solves a seeded sample of cubes with each setting of the knobs and keeps the
+fastest: nodes per fetched chunk (node budget), workgroups per CU, prefix
+length, and the per-round prefix policy (the node counts at which a round's
+items get one and two moves longer).  Probes in flight per lane are fixed by
+the kernel (one forward and one inverse frame per step).  Every setting gives
+the same optimal lengths; the tuner checks that too.
+
+    python -m cuda_cube_solver_amd.tune --table d4t --cubes 512 --seed 33 [--out knobs.json]
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import sys
+import time
+
+from . import states
+from .solver import GpuSolver, Tables
+
+CHUNK_NODES = (65536, 262144, 1048576)
+BLOCKS_PER_CU = (0, 2)       # 0: as many as the occupancy calculator allows
+PREFIX_LEN = (4, 5)
+DEEPER_AT = ((5e7, 3e9), (1e7, 1e9))
+
+
+def tune(tables: Tables, cubes, log=None) -> dict:
+    """Best knobs for solving `cubes` on `tables` in the batched form."""
+    trials, lengths = [], None
+
+    def run(chunk, bpc, plen, deeper):
+        nonlocal lengths
+        solver = GpuSolver(table=tables, mode="batched", prefix_len=plen, deeper_prefix_at=deeper,
+                           blocks_per_cu=bpc, chunk_nodes=chunk)
+        t = time.perf_counter()
+        sols = solver.solve(cubes)
+        dt = time.perf_counter() - t
+        got = [len(s.moves) for s in sols]
+        if lengths is None:
+            lengths = got
+        elif got != lengths:
+            raise AssertionError(f"knobs {chunk, bpc, plen, deeper} changed the optimal lengths")
+        rec = {"chunk_nodes": chunk, "blocks_per_cu": bpc, "prefix_len": plen, "deeper_prefix_at": list(deeper),
+               "cubes_per_s": round(len(cubes) / dt, 2)}
+        trials.append(rec)
+        if log:
+            log(f"tune: {rec}")
+        return rec
+
+    for chunk, bpc, plen in itertools.product(CHUNK_NODES, BLOCKS_PER_CU, PREFIX_LEN):
+        run(chunk, bpc, plen, DEEPER_AT[0])
+    best = max(trials, key=lambda r: r["cubes_per_s"])
+    for deeper in DEEPER_AT[1:]:
+        run(best["chunk_nodes"], best["blocks_per_cu"], best["prefix_len"], deeper)
+    best = max(trials, key=lambda r: r["cubes_per_s"])
+    return {"best": best, "trials": trials, "sample": len(cubes)}
+
+
+def knobs(best: dict) -> dict:
+    """GpuSolver keyword arguments of a tuner result."""
+    return {"chunk_nodes": best["chunk_nodes"], "blocks_per_cu": best["blocks_per_cu"],
+            "prefix_len": best["prefix_len"], "deeper_prefix_at": tuple(best["deeper_prefix_at"])}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="cuda_cube_solver_amd.tune")
+    ap.add_argument("--table", default="d4t")
+    ap.add_argument("--cubes", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=33)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    tb, _ = Tables.build(args.table, "block")
+    try:
+        res = tune(tb, states.uniform_states(args.cubes, args.seed),
+                   log=lambda m: print(m, file=sys.stderr, flush=True))
+    finally:
+        tb.close()
+    text = json.dumps(res, indent=1)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text + "\n")
+    print(json.dumps(res["best"]))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

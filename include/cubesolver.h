@@ -165,6 +165,9 @@ typedef struct {
     int32_t prefix_len;          /* as in ccs_solve_opts_t */
     double deeper_at[2];
     int32_t blocks_per_cu;
+    int32_t chunk_nodes;         /* target nodes per chunk of items a lane takes (0: 16384); a
+                                    lane covers ~40k nodes/s, so this bounds how long one chunk
+                                    holds a cube back */
 } ccs_server_opts_t;
 
 typedef struct {

@@ -1,0 +1,153 @@
+"""GPU parity tests for the benchmark workloads and the reference's hard
+positions on d4: optimal lengths against the reference's recorded results or
+the oracle's fixtures, nodes in completed rounds against the reference's own
+figures (research/subgroups/hard.jsonl) or the oracle's.
+
+  C1  64 uniform states, seed 1, table 304, plain form one cube at a time
+  C2  1,000 uniform states, seed 2, d4t
+  C3  4,096 uniform states, seed 3, d4t (first 64 here; bench.py runs all)
+  C4  d4: 256 of the reference's length >= 18 positions drawn with seed 4,
+      plus 64 flip-heavy states (seed 4)
+"""
+import json
+from pathlib import Path
+
+import pytest
+
+from cuda_cube_solver_amd import states
+from cuda_cube_solver_amd.cubie import MOVES_HTM, CubieState, apply_moves, parse_moves
+from cuda_cube_solver_amd.solver import GpuSolver, Tables
+
+pytestmark = pytest.mark.gpu
+
+
+def _scrambled(texts):
+    return [apply_moves(CubieState(), [MOVES_HTM[m] for m in parse_moves(t)]) for t in texts]
+
+
+def _complete_nodes(sol):
+    return sum(n for b, n in sol.iteration_nodes if b < len(sol.moves))
+
+
+def _solves(cubes, sols):
+    return all(apply_moves(c, [MOVES_HTM[m] for m in s.moves]) == CubieState() for c, s in zip(cubes, sols))
+
+
+@pytest.fixture(scope="module")
+def seeded():
+    return json.loads((Path(__file__).resolve().parent / "golden" / "seeded_results.json").read_text())
+
+
+@pytest.fixture(scope="module")
+def td4(reference_results, table_hashes):
+    tb, counts = Tables.build("d4", "block")
+    assert counts == reference_results["depth_counts"]["d4"]["depth_counts"]
+    yield tb
+    tb.close()
+
+
+@pytest.fixture(scope="module")
+def td4t(reference_results):
+    tb, counts = Tables.build("d4t", "block")
+    assert counts == reference_results["depth_counts"]["d4t"]["depth_counts"]
+    yield tb
+    tb.close()
+
+
+def test_d4_table_is_bit_exact(td4, table_hashes):
+    import hashlib
+
+    want = table_hashes["d4"]
+    assert td4.nbytes("block") == want["blk_bytes"]
+    h = hashlib.sha256()
+    for off in range(0, want["blk_bytes"], 1 << 28):
+        h.update(td4.read("block", off, min(1 << 28, want["blk_bytes"] - off)))
+    assert h.hexdigest() == want["blk_sha256"]
+
+
+def test_d4_reference_positions_and_node_counts(td4, reference_results):
+    """First 100 random positions and the superflip on d4 (g1-26): the
+    reference's lengths and its completed-round node totals."""
+    ref = reference_results["first100_d4"]
+    nodes = reference_results["complete_round_nodes"]["d4"]
+    cubes = _scrambled(reference_results["random1000"]["scrambles"][:100])
+    solver = GpuSolver(table=td4, mode="plain")
+    sols = solver.solve(cubes)
+    assert [len(s.moves) for s in sols] == [r["len"] for r in ref] == nodes["random_len"]
+    assert sum(_complete_nodes(s) for s in sols) == round(nodes["random_nodes"] * 100)
+    sf = _scrambled(reference_results["superflip"]["scrambles"])
+    sol = solver.solve(sf)[0]
+    assert len(sol.moves) == 20 and _solves(sf, [sol])
+    assert _complete_nodes(sol) == round(nodes["superflip_nodes"])
+
+
+def test_c4_hard_tail_on_d4(td4, td4t, reference_results):
+    """256 of the reference's length >= 18 positions (seed 4) in one batch on d4:
+    the reference's lengths.  64 flip-heavy states: the same lengths on d4 and
+    on d4t (any admissible table gives the optimum), the superflip at 20."""
+    ref = reference_results["random1000"]
+    pool = [i for i, r in enumerate(ref["results"]) if r["len"] >= 18]
+    assert len(pool) == 692
+    pick = [pool[i] for i in states.sample_indices(len(pool), 256, 4)]
+    cubes = _scrambled([ref["scrambles"][i] for i in pick])
+    sols = GpuSolver(table=td4, mode="batched").solve(cubes)
+    assert [len(s.moves) for s in sols] == [ref["results"][i]["len"] for i in pick]
+    assert _solves(cubes, sols)
+    flips = states.flip_heavy_states(64, 4)
+    on_d4 = GpuSolver(table=td4, mode="batched").solve(flips)
+    on_d4t = GpuSolver(table=td4t, mode="batched").solve(flips)
+    assert [len(s.moves) for s in on_d4] == [len(s.moves) for s in on_d4t]
+    assert len(on_d4[-1].moves) == 20
+    assert _solves(flips, on_d4) and _solves(flips, on_d4t)
+
+
+def test_d4t_distance_20_node_counts(td4t, reference_results):
+    """The reference's ten distance-20 positions on d4t (g1-27): completed-round
+    nodes sum to the reference's figure, in both search forms."""
+    d20 = reference_results["distance20_random10"]
+    want = round(reference_results["complete_round_nodes"]["d4t"]["d20_nodes"] * 10)
+    cubes = _scrambled(d20["scrambles"])
+    for mode in ("plain", "batched"):
+        sols = GpuSolver(table=td4t, mode=mode).solve(cubes)
+        assert [len(s.moves) for s in sols] == [20] * 10
+        assert sum(_complete_nodes(s) for s in sols) == want
+
+
+def test_c2_seed2_on_d4t(td4t, seeded):
+    """1,000 seed-2 states: the batched form equals the plain form (lengths and
+    completed-round nodes per cube); the first 64 equal the oracle's."""
+    cubes = states.uniform_states(1000, 2)
+    solver = GpuSolver(table=td4t)
+    batched = solver.solve(cubes, mode="batched")
+    plain = solver.solve(cubes, mode="plain")
+    assert [len(s.moves) for s in batched] == [len(s.moves) for s in plain]
+    assert [_complete_nodes(s) for s in batched] == [_complete_nodes(s) for s in plain]
+    fx = seeded["c2_d4t_seed2_first64"]
+    assert [len(s.moves) for s in batched[:64]] == fx["lengths"]
+    assert [_complete_nodes(s) for s in batched[:64]] == fx["complete_round_nodes"]
+    assert _solves(cubes, batched)
+
+
+def test_c3_seed3_first64_on_d4t(td4t, seeded):
+    cubes = states.uniform_states(64, 3)
+    sols = GpuSolver(table=td4t, mode="batched").solve(cubes)
+    fx = seeded["c3_d4t_seed3_first64"]
+    assert [len(s.moves) for s in sols] == fx["lengths"]
+    assert [_complete_nodes(s) for s in sols] == fx["complete_round_nodes"]
+
+
+def test_c1_seed1_on_304_plain_one_at_a_time(seeded, reference_results):
+    """C1: 64 seed-1 states on 304, each solved alone in the plain form (one
+    launch per IDA* round): the oracle's lengths and completed-round nodes."""
+    tb, counts = Tables.build("304", "block")
+    try:
+        assert counts == reference_results["depth_counts"]["304"]["depth_counts"]
+        cubes = states.uniform_states(64, 1)
+        solver = GpuSolver(table=tb, mode="plain")
+        sols = [solver.solve([c])[0] for c in cubes]
+    finally:
+        tb.close()
+    fx = seeded["c1_304_seed1"]
+    assert [len(s.moves) for s in sols] == fx["lengths"]
+    assert [_complete_nodes(s) for s in sols] == fx["complete_round_nodes"]
+    assert _solves(cubes, sols)
