@@ -1,8 +1,13 @@
 """Benchmark: cubes/s on the d4t table, persistent batched form vs plain one-cube-at-a-time.
 
 One step = one batched solve of the workload (4,096 seeded uniform random
-states on d4t, one persistent launch).  The d4t table is built on the GPU
-first (BFS kernels, not timed).  Prints one JSON line.
+states, seed 3, on d4t, one persistent launch).  Before it: the d4t table is
+built on the GPU (BFS kernels) and the autotuner picks the persistent form's
+knobs on a separate 512-state sample (seed 33); neither is timed.  After it:
+the first 128 states solved one at a time in the plain form, the random-probe
+ceiling of the memory system, a Poisson serving run at 50% and 80% of the
+measured capacity (seed 5), and the CPU oracle on a bounded sample.  Prints
+one JSON line.
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1]
 """
@@ -146,7 +151,11 @@ def main() -> None:
         log(f"step {k + 1}/{args.steps}: {step_s[-1]:.2f}s, {st.nodes / 1e9:.2f} G nodes, "
             f"{st.probes / 1e9:.2f} G probes")
     total_s = sum(step_s)
-    value = args.cubes * args.steps / total_s
+    # value: the device time of the search launches (HIP events on the library's
+    # stream), inputs already in HBM; the end-to-end rate adds host root
+    # preparation and the 40-byte-per-cube copies over PCIe
+    value = args.cubes * args.steps / (kernel_ms / 1e3)
+    end_to_end = args.cubes * args.steps / total_s
 
     # plain: one cube at a time, per-round launches (the reference's form at batch 1)
     plain_ms = []
@@ -169,7 +178,7 @@ def main() -> None:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(total_s / args.steps * 1e3, 1),
+        "ms_per_step": round(kernel_ms / args.steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -186,7 +195,9 @@ def main() -> None:
         "plain_p50_ms": round(pct(plain_ms, 50), 2) if plain_ms else None,
         "plain_p99_ms": round(pct(plain_ms, 99), 2) if plain_ms else None,
         "plain_cubes": len(plain_ms),
-        "batched_over_plain": round(value / plain_value, 2) if plain_ms else None,
+        "end_to_end_cubes_per_s": round(end_to_end, 3),
+        "end_to_end_ms_per_step": round(total_s / args.steps * 1e3, 1),
+        "batched_over_plain": round(end_to_end / plain_value, 2) if plain_ms else None,
         "lengths_identical": same,
         "length_histogram": {str(L): lengths.count(L) for L in sorted(set(lengths))},
         "probes_per_cube": round(probes / (args.cubes * args.steps)),
@@ -202,7 +213,7 @@ def main() -> None:
         hard = [c for c, L in zip(cubes, lengths) if L >= 18]
         line["serving"] = []
         for frac in (0.5, 0.8):
-            qps = frac * value
+            qps = frac * end_to_end
             r = serve_bench.run(tb, qps, max(16, int(qps * args.serve_seconds)), seed=5, hard_pool=hard, log=log)
             r.pop("lengths")
             r["load"] = f"{frac:.0%} of the batched capacity"
