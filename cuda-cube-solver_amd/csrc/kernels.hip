@@ -663,7 +663,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     uint32_t hp = 0, cp = 0;
     uint32_t c = 0xFFFFFFFFu;  // cube of the current item
     SearchJob* J = nullptr;
-    uint32_t bound = 0, todo = 0, ord = 0, iter = 0, idle = 0, plen = 0;
+    uint32_t bound = 0, todo = 0, ord = 0, idle = 0, plen = 0;
     int g = 0, tgt = 0, m = -1;
     // Child m under evaluation: forward / inverse frames still to probe (bits
     // in frame_order's layout), whether a probe pruned it, whether it is done.
@@ -672,8 +672,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // Phase of the lane's item: 0 fetch one, 1 undo down to the prefix shared
     // with the previous item, 2 replay the prefix, 3 search below it, 4 done.
     int phase = 0;
-    unsigned long long nodes = 0, probes = 0, all_nodes = 0, idle_since = 0, chunk_nodes = 0;
+    unsigned long long nodes = 0, idle_since = 0, chunk_nodes = 0;
+    uint32_t probes = 0;  // added to the launch's counter as it nears 2^31
     unsigned long long pbits = 0;  // the item's prefix, move j in byte j
+    // Persistent form: the next item's prefix and the cube's found flag, both
+    // read one item ahead so that no item start waits on a global load.
+    unsigned long long pnext = 0;
+    uint32_t found_seen = 0;
     uint32_t chunk_left = 0, item_cur = 0;  // persistent form: rest of this lane's chunk
     // The lane's frames stay at depth `hold` on the previous item's prefix
     // (-1: nothing held); `hfail` is the depth of that item's failing prefix
@@ -717,16 +722,19 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             }
             bool pop = phase == 1;
             if (phase == 3 && todo == 0) {  // node exhausted: the item is complete, or back up
-                if ((uint32_t)g == plen || ld_c(&J->found)) phase = 4;
+                if ((uint32_t)g == plen || found_seen) phase = 4;
                 else pop = true;
             }
             if (phase == 4) {
                 // ---- the item is finished: account its nodes ----
                 hold = (uint32_t)g <= plen ? g : -1;  // deeper only when found ended the search
-                all_nodes += nodes;
+                if (probes >= 0x80000000u && A.stats) {
+                    atomicAdd(&A.stats[0], (unsigned long long)probes);
+                    probes = 0;
+                }
                 if (QUEUE) {
                     chunk_nodes += nodes;
-                    if (!chunk_left || ld_c(&J->found)) {
+                    if (!chunk_left || found_seen) {
                         chunk_left = 0;
                         end_chunk(A, c, J, chunk_nodes);
                     }
@@ -756,6 +764,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     c = cw;
                     J = &A.jobs[c];
                     if (ld_c(&J->found)) continue;
+                    found_seen = 0;
                     bound = J->bound;
                     plen = (uint32_t)A.plen;
                 } else if (chunk_left) {
@@ -763,11 +772,6 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     --chunk_left;
                     item = ++item_cur;
                     fresh = false;
-                    if (ld_c(&J->found)) {
-                        chunk_left = 0;
-                        end_chunk(A, c, J, chunk_nodes);
-                        continue;
-                    }
                 } else {
                     if (idle && (--idle)) break;  // back off between empty fetches
                     unsigned long long mask = __ballot(1);
@@ -841,7 +845,14 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 }
                 // ---- start the item: its prefix is replayed move by move through
                 // the probe step below, all six frames (kernels.cu:1552-1614) ----
-                const unsigned long long nb = (QUEUE ? A.pre_tab[plen] : A.prefixes)[item];
+                unsigned long long nb;
+                if (QUEUE) {
+                    nb = fresh ? A.pre_tab[plen][item] : pnext;
+                    if (chunk_left) pnext = A.pre_tab[plen][item + 1u];  // used by the next item
+                    found_seen = ld_c(&J->found);  // looked at when this item ends
+                } else {
+                    nb = A.prefixes[item];
+                }
                 fresh = fresh || hold < 0;
                 unsigned long long x = nb ^ pbits;
                 int shared = fresh ? 0 : min((int)plen, x ? (int)(__builtin_ctzll(x) >> 3) : 8);
@@ -907,10 +918,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 break;
             }
             if (phase == 3 && todo) {
-                if (((++iter & A.poll_mask) == 0) && ld_c(&J->found)) {
+                if (found_seen) {  // another lane solved the cube
                     phase = 4;
                     continue;
                 }
+                // refresh the flag every poll_mask + 1 picks; it is read from the next pick on,
+                // so the load's latency hides behind this step's probes
+                if ((nodes & A.poll_mask) == 0) found_seen = ld_c(&J->found);
                 // ---- pick the next canonical child ----
                 m = (int)ctz(todo);
                 todo &= todo - 1;
@@ -967,14 +981,9 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         }
     }
     if (A.stats) {
-        for (int o = 32; o > 0; o >>= 1) {
-            probes += __shfl_xor(probes, o);
-            all_nodes += __shfl_xor(all_nodes, o);
-        }
-        if (lane_id() == 0) {
-            atomicAdd(&A.stats[0], probes);
-            atomicAdd(&A.stats[1], all_nodes);
-        }
+        unsigned long long p = probes;
+        for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o);
+        if (lane_id() == 0) atomicAdd(&A.stats[0], p);
     }
 }
 

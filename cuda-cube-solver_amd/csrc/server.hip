@@ -214,7 +214,7 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     A.max_bound = STACK_DEPTH;
     A.deeper0 = (unsigned long long)o.deeper_at[0];
     A.deeper1 = (unsigned long long)o.deeper_at[1];
-    A.poll_mask = 1023;
+    A.poll_mask = 63;  // found flag refreshed every 64 picks (read one pick later)
     A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : SERVE_CHUNK_NODES;
     A.idle_limit = ~0ull;  // idling is normal while serving
     A.serving = 1;

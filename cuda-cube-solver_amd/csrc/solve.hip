@@ -48,7 +48,7 @@ void fill_opts(const ccs_solve_opts_t* in, ccs_solve_opts_t& o) {
     if (o.prefix_len == 0) o.prefix_len = 4;
     if (o.block == 0) o.block = SEARCH_WG;
     if (o.max_bound == 0) o.max_bound = MAX_BOUND;
-    if (o.poll_interval == 0) o.poll_interval = 1024;
+    if (o.poll_interval == 0) o.poll_interval = 64;
     if (o.deeper_at[0] == 0 && o.deeper_at[1] == 0) o.deeper_at[0] = 5e7, o.deeper_at[1] = 3e9;
 }
 
@@ -364,7 +364,8 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
         unsigned long long hs[4];
         HIP_TRY(hipMemcpy(hs, d_stats, sizeof hs, hipMemcpyDeviceToHost));
         stats->probes = hs[0];
-        stats->nodes = hs[1];
+        stats->nodes = 0;  // children evaluated plus prefix moves replayed, every round of every cube
+        for (int i = 0; i < n; ++i) stats->nodes += out[i].nodes;
         stats->kernel_ms = kernel_ms;
         stats->launches = launches;
         stats->wall_ms = now_ms() - t_start;

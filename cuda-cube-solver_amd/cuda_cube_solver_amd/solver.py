@@ -150,7 +150,7 @@ class GpuSolver:
 
     def __init__(self, tables: Path = Path("tables"), kind: str = "d4t", table_format: str = "block",
                  mode: str = "plain", prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0,
-                 poll_interval: int = 1024, chunk_nodes: int = 0, table: Tables | None = None):
+                 poll_interval: int = 64, chunk_nodes: int = 0, table: Tables | None = None):
         if table is None:
             tables = Path(tables)
             if tables.suffix == ".bin":

@@ -127,7 +127,7 @@ typedef struct {
     int32_t block;          /* workgroup size (0: 256) */
     int32_t blocks_per_cu;  /* persistent grid per CU (0: from occupancy) */
     int32_t max_bound;      /* give up above this bound (0: 24); searches stop at 20, the longest optimal solution of any legal cube */
-    int32_t poll_interval;  /* picks between polls of the found flag, power of two (0: 1024) */
+    int32_t poll_interval;  /* picks between polls of the found flag, power of two (0: 64) */
     int32_t chunk_nodes;    /* persistent form: target nodes per chunk of items a lane takes (0: 262144) */
 } ccs_solve_opts_t;
 
