@@ -635,6 +635,9 @@ void build_lut(const Kind& k, CubeLUT& L) {
     for (int i = 0; i < 495; ++i) L.slice_unrank[i] = S.slice_unrank[i];
     for (int i = 0; i < 24; ++i) L.eorder_unrank[i] = S.eorder_unrank[i];
     memcpy(L.gen_class, k.gen_class, sizeof L.gen_class);
+    for (int key = 0; key < 256; ++key)
+        if (S.eorder_rank[key] != 0xFF)
+            for (int fl = 0; fl < 16; ++fl) L.key_class[key * 16 + fl] = k.gen_class[S.eorder_rank[key] * 16 + fl];
     memcpy(L.gen_rep, k.gen_rep, sizeof L.gen_rep);
     memcpy(L.s_mul, S.s_mul, sizeof L.s_mul);
     for (int b = 0; b < 243; ++b)

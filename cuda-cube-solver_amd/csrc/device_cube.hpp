@@ -165,7 +165,7 @@ CCS_HD uint32_t edge_coord(const CubeLUT& L, const Frame& f, uint32_t s) {
         uint32_t lab = ew & 0x03030303u;
         uint32_t key = (lab | (lab >> 6) | (lab >> 12) | (lab >> 18)) & 0xFFu;
         uint32_t nflip = gather4(nlo, 4) | (gather4(nhi, 4) << 4);
-        uint32_t c = L.gen_class[(uint32_t)L.eorder_rank[key] * 16u + eflip];
+        uint32_t c = L.key_class[key * 16u + eflip];
         return mul24(slice * 256u + nflip, (uint32_t)K) + c;
     }
 }

@@ -51,6 +51,7 @@ struct alignas(16) CubeLUT {
     uint8_t ccomb_rank[256];  // 8C4 lexicographic rank
     uint8_t eorder_rank[256]; // 4 two-bit labels (first in low bits) -> permutation rank
     uint8_t gen_class[384];   // group kinds: E content (order rank * 16 + flips) -> class
+    uint8_t key_class[256 * 16];  // the same by order key (four 2-bit labels) * 16 + flips: one lookup
     uint8_t s_mul[16][16];    // index of M_t M_s among the UD symmetries
     uint8_t ccomb_unrank[72];
     uint16_t slice_unrank[496];
