@@ -117,13 +117,28 @@ CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) { left_mul(f, L.lr[1][
 
 CCS_HD int inverse_move(int m) { return 3 * (m / 3) + 2 - m % 3; }
 
+// Byte i (0..15) of a 16-byte table held in four words, by v_perm.
+CCS_HD uint32_t byte16(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t i) {
+    const uint32_t lo = vperm(w1, w0, i & 7u), hi = vperm(w3, w2, i & 7u);
+    return ((i & 8u) ? hi : lo) & 0xFFu;
+}
+// Lexicographic rank (itertools.combinations order) of a 4-of-8 position mask.
+// The positions below 4 are a prefix of the sorted tuple, so the tuples with
+// one low part are contiguous and rank = TL[low nibble] + TH[high nibble],
+// TH being the rank of the high part among subsets of its size.  Equal to the
+// ccomb_rank table on all 70 masks; no LDS round trip before the classidx read.
+CCS_HD uint32_t comb8_rank(uint32_t m) {
+    return byte16(0x09331F45u, 0x0127133Du, 0x052D1941u, 0x00230F37u, m & 15u) +
+           byte16(0x00010000u, 0x00030102u, 0x01040203u, 0x00030205u, m >> 4);
+}
+
 // twist (3^7) * 70 + rank of the D-corner positions (coords.py:112-118).
 CCS_HD uint32_t corner_raw(const CubeLUT& L, const Frame& f) {
     uint32_t olo = (f.c0 >> 3) & 0x03030303u, ohi = (f.c1 >> 3) & 0x00030303u;
     // sum of co[i] * 3^i over corners 0..6 (corner 7 is fixed by the others)
     uint32_t twist = dot4(olo, 0x1B090301u, mul24(dot4(ohi, 0x00090301u, 0u), 81u));
     uint32_t dmask = dot4((f.c0 >> 2) & 0x01010101u, 0x08040201u, dot4((f.c1 >> 2) & 0x01010101u, 0x80402010u, 0u));
-    return mul24(twist, 70u) + L.ccomb_rank[dmask];
+    return mul24(twist, 70u) + comb8_rank(dmask);
 }
 
 // K = 0: the 304 kind; K > 0: a group kind with K E-content classes.
@@ -160,7 +175,7 @@ CCS_HD uint32_t edge_coord(const CubeLUT& L, const Frame& f, uint32_t s) {
     if constexpr (K == 0) {
         // U-layer edges are cubies 0..3 (bits 2 and 3 clear)
         uint32_t umask = gather4(~(nlo | (nlo >> 1)), 2) | (gather4(~(nhi | (nhi >> 1)), 2) << 4);
-        return (mul24(slice, 70u) + L.ccomb_rank[umask]) * 16u + eflip;
+        return (mul24(slice, 70u) + comb8_rank(umask)) * 16u + eflip;
     } else {
         uint32_t lab = ew & 0x03030303u;
         uint32_t key = (lab | (lab >> 6) | (lab >> 12) | (lab >> 18)) & 0xFFu;

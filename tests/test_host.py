@@ -56,6 +56,37 @@ def test_kernel_lut_arithmetic_matches_oracle_on_seeded_states():
         assert (out == O.table_index(O.kind(name), arr)).all()
 
 
+def test_kernel_combination_ranks_cover_every_placement():
+    """Every placement of the four D corners (the corner coordinate's 8C4 rank)
+    and of the four U edges among the non-slice positions (304's 8C4 rank),
+    with parities fixed up by swapping two other cubies: kernel arithmetic
+    equals the oracle's index."""
+    from itertools import combinations, permutations
+
+    sts = []
+    for k, pos in enumerate(combinations(range(8), 4)):
+        cp = [0] * 8
+        d_iter, u_iter = iter((4, 5, 6, 7)), iter((0, 1, 2, 3))
+        for i in range(8):
+            cp[i] = next(d_iter) if i in pos else next(u_iter)
+        non_e = [0, 1, 2, 3, 4, 5, 6, 7]  # positions of the non-slice edges (8..11 are the E slice)
+        upos = list(combinations(non_e, 4))[k]
+        ep = [0] * 12
+        ui, di = iter((0, 1, 2, 3)), iter((4, 5, 6, 7))
+        for i in non_e:
+            ep[i] = next(ui) if i in upos else next(di)
+        ep[8:] = [8, 9, 10, 11]
+        if cubie.permutation_parity(cp) != cubie.permutation_parity(ep):
+            ep[10], ep[11] = ep[11], ep[10]
+        sts.append(CubieState(tuple(cp), (0,) * 8, tuple(ep), (0,) * 12))
+    assert all(s.is_legal() for s in sts) and len(sts) == 70
+    arr = O.cubes([(s.cp, s.co, s.ep, s.eo) for s in sts])
+    for name in ("304", "d4t"):
+        out = np.zeros(len(arr), dtype=np.uint64)
+        _lib.check(_lib.lib().ccs_index_host(name.encode(), arr.ctypes.data, len(arr), out.ctypes.data))
+        assert (out == O.table_index(O.kind(name), arr)).all()
+
+
 def test_kind_info():
     for name in ("304", "d4t", "d4", "s4", "d4f", "a4t", "a4"):
         k = _lib.kind_info(name)
