@@ -946,6 +946,10 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         if (eval) {
             // ---- one step of child m: its next forward and next inverse frame,
             // both table reads in flight together ----
+            // raised wave priority until the table reads are issued: a wave that is
+            // ready to compute addresses goes ahead of waves doing bookkeeping, so
+            // more misses overlap
+            __builtin_amdgcn_s_setprio(2);
             bool hasF = remF != 0u, hasI = remI != 0u;
             if (first) {  // a child's first step probes only its top frame, where most are pruned
                 if ((ctz(remF | 0x10000u) >> 2) <= (ctz(remI | 0x10000u) >> 2)) hasI = false;
@@ -962,6 +966,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             uint32_t jF, jI;
             const uint8_t* bF = probe_block<K, F>(L, pkF, table, tf, jF);
             const uint8_t* bI = probe_block<K, F>(L, pkI, table, ti, jI);
+            __builtin_amdgcn_s_setprio(0);
             uint32_t vf = hasF ? block_value<F>(L, bF, jF) : 0u;
             const uint32_t vi = hasI ? block_value<F>(L, bI, jI) : 0u;
             probes += (uint32_t)hasF + (uint32_t)hasI;
