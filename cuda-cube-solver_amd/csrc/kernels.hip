@@ -946,9 +946,9 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         if (eval) {
             // ---- one step of child m: its next forward and next inverse frame,
             // both table reads in flight together ----
-            // raised wave priority until the table reads are issued: a wave that is
-            // ready to compute addresses goes ahead of waves doing bookkeeping, so
-            // more misses overlap
+            // raised wave priority from here until the table addresses are known: a
+            // wave computing a probe goes ahead of waves doing bookkeeping, so more
+            // table misses overlap
             __builtin_amdgcn_s_setprio(2);
             bool hasF = remF != 0u, hasI = remI != 0u;
             if (first) {  // a child's first step probes only its top frame, where most are pruned
