@@ -26,11 +26,11 @@ sys.path.insert(0, str(ROOT / "cuda-cube-solver_amd"))
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PROBE = 32  # algorithmic: one 32-byte table block per lookup
 # HBM bytes moved per probe by the search kernel, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch) scaled by a 256 MiB copy kernel's readings
-# (FETCH_SIZE x2, WRITE_SIZE x1): profiles/search_kernel_d4t_pmc.md.  A random
-# 32-byte block costs one 128-byte line.
-FETCHED_BYTES_PER_PROBE = 128.6
-WRITTEN_BYTES_PER_PROBE = 0.04
+# WRITE_SIZE (KiB per dispatch) scaled by kernels of known size in the same run
+# (FETCH_SIZE x2: it books each 128-byte read request as 64 B; WRITE_SIZE x1):
+# profiles/search_kernel_d4t_pmc.md.  A random 32-byte block costs one 128-byte line.
+FETCHED_BYTES_PER_PROBE = 128.0
+WRITTEN_BYTES_PER_PROBE = 0.035
 
 
 def log(msg: str) -> None:
