@@ -115,8 +115,6 @@ CCS_HD void left_mul(Frame& f, const LeftRec& r) {
 CCS_HD void apply_move(const CubeLUT& L, Frame& f, int m) { right_mul(f, L.rr[0][0][m]); }
 CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) { left_mul(f, L.lr[1][0][m]); }
 
-CCS_HD int inverse_move(int m) { return 3 * (m / 3) + 2 - m % 3; }
-
 // Byte i (0..15) of a 16-byte table held in four words, by v_perm.
 CCS_HD uint32_t byte16(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t i) {
     const uint32_t lo = vperm(w1, w0, i & 7u), hi = vperm(w3, w2, i & 7u);

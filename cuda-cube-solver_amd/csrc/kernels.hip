@@ -432,7 +432,7 @@ __device__ __forceinline__ unsigned long long make_ticket(unsigned long long kee
     return (keep & T_INF_MASK) | ((unsigned long long)bound << T_BOUND_SHIFT) |
            ((unsigned long long)plen << T_PLEN_SHIFT) | ((unsigned long long)chunk_log << T_CHUNK_SHIFT);
 }
-// Items per fetch for a round: about chunk_nodes nodes per chunk, judged by
+// Items per fetch for a round: about chunk_nodes nodes per chunk, estimated from
 // the previous round's nodes and the usual growth of ~13x per bound, but at
 // least enough chunks for this cube's share of the launch's lanes (and never
 // fewer than 4096), so that the last cubes of a batch still fill the GPU.
@@ -572,12 +572,6 @@ __device__ __forceinline__ void report_solution(const SearchArgs& A, SearchJob* 
     }
 }
 
-template <int K, int F>
-__device__ __forceinline__ uint32_t frame_lb_(const CubeLUT& L, const uint32_t* __restrict__ classidx,
-                                              const uint8_t* __restrict__ table, const Frame& f) {
-    return frame_bound<K, F>(L, classidx, table, f);
-}
-
 // Serving: take one mailbox record into its slot: root bounds of the six
 // frames, a fresh round ticket (keeping the in-flight count of late
 // fetchers), and a ring entry.  Returns false when the mailbox is empty.
@@ -600,7 +594,7 @@ __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
         st_c(&dst[2], f.e0);
         st_c(&dst[3], f.e1);
         st_c(&dst[4], f.e2);
-        uint32_t v = frame_lb_<K, F>(L, A.classidx, A.table, f);
+        uint32_t v = frame_bound<K, F>(L, A.classidx, A.table, f);
         st_c(&J->h[j], v);
         bound = max(bound, v);
     }
@@ -623,12 +617,6 @@ __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
     unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
     st_c(&A.ring[slot % A.ring_size], (slot << 32) | c);
     return true;
-}
-
-template <int K, int F>
-__device__ __forceinline__ uint32_t frame_lb(const CubeLUT& L, const uint32_t* __restrict__ classidx,
-                                             const uint8_t* __restrict__ table, const Frame& f) {
-    return frame_bound<K, F>(L, classidx, table, f);
 }
 
 // A lane finished its chunk of items: add its nodes to the round, wait for
