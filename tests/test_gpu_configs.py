@@ -151,3 +151,19 @@ def test_c1_seed1_on_304_plain_one_at_a_time(seeded, reference_results):
     assert [len(s.moves) for s in sols] == fx["lengths"]
     assert [_complete_nodes(s) for s in sols] == fx["complete_round_nodes"]
     assert _solves(cubes, sols)
+
+
+@pytest.mark.parametrize("kind", ["s4", "d4f", "a4t", "a4"])
+def test_other_presets_bfs_depth_counts(kind, reference_results):
+    """The remaining group kinds the presets use: the GPU BFS reproduces the
+    reference's depth counts (research/subgroups/results.jsonl) and the table
+    passes the self-checks; a few positions solve to the reference's lengths."""
+    tb, counts = Tables.build(kind, "block")
+    try:
+        assert counts == reference_results["depth_counts"][kind]["depth_counts"]
+        ref = reference_results["random1000"]
+        cubes = _scrambled(ref["scrambles"][:8])
+        sols = GpuSolver(table=tb, mode="batched").solve(cubes)
+        assert [len(s.moves) for s in sols] == [r["len"] for r in ref["results"][:8]]
+    finally:
+        tb.close()
