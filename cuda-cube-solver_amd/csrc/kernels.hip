@@ -420,7 +420,33 @@ constexpr unsigned long long T_CLOSING = 1ull << 57;
 constexpr unsigned long long T_RETIRED = 1ull << 58;  // serving: the cube's slot is free again
 constexpr int T_CHUNK_SHIFT = 59;
 // Serving: a fetching wave admits new cubes while fewer rounds than this are queued.
-constexpr unsigned long long ADMIT_BELOW = 256;                     // log2 of the items a lane takes per fetch (0..6)
+constexpr unsigned long long ADMIT_BELOW = 256;
+// Ring entry: (slot mod 2^27) << 37 | round bound << 32 | cube.  The slot part
+// tells a reader that the entry at its head has been published; the bound
+// tells a stale entry (of a round that has since closed) from the cube's live one.
+constexpr unsigned long long RING_SEQ_MASK = (1ull << 27) - 1;
+__device__ __forceinline__ unsigned long long ring_entry(unsigned long long slot, uint32_t bound, uint32_t cube) {
+    return ((slot & RING_SEQ_MASK) << 37) | ((unsigned long long)bound << 32) | cube;
+}
+__device__ __forceinline__ void ring_push(const SearchArgs& A, uint32_t bound, uint32_t cube) {
+    unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
+    st_c(&A.ring[slot % A.ring_size], ring_entry(slot, bound, cube));
+}
+// Chunk layout of the persistent form.  Chunk q of a round with 2^lg items per
+// chunk and Q = ceil(n / 2^lg) chunks takes blocks of ITEM_BLOCK consecutive
+// items spaced Q blocks apart (blocks q, q + Q, q + 2Q, ...).  The lanes that
+// hold a cube's chunks then sweep its items roughly in order, so the round that
+// finds the solution stops near where a sequential search would; with
+// contiguous chunks every chunk was the same fraction done when the solution
+// turned up, half the round on average.  Consecutive items within a block
+// still share their leading prefix moves.
+constexpr uint32_t ITEM_BLOCK = 8;
+__device__ __forceinline__ uint32_t next_item(uint32_t item, uint32_t left, uint32_t lg, uint32_t n) {
+    const uint32_t blk = min(ITEM_BLOCK, 1u << lg);
+    if (left & (blk - 1u)) return item + 1u;  // within a block
+    const uint32_t q = (n + (1u << lg) - 1u) >> lg;
+    return item + (q - 1u) * blk + 1u;        // first item of the chunk's next block
+}                     // log2 of the items a lane takes per fetch (0..6)
 
 __device__ __forceinline__ uint32_t t_next(unsigned long long t) { return (uint32_t)(t & T_NEXT_MASK); }
 __device__ __forceinline__ uint32_t t_inflight(unsigned long long t) { return (uint32_t)((t & T_INF_MASK) >> 28); }
@@ -480,8 +506,7 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
         if (prev == cur) break;
         cur = prev;
     }
-    unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
-    st_c(&A.ring[slot % A.ring_size], (slot << 32) | c);
+    ring_push(A, nb, c);
 }
 
 __device__ __forceinline__ bool round_drained(const SearchArgs& A, unsigned long long t) {
@@ -614,8 +639,7 @@ __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
         if (prev == cur) break;
         cur = prev;
     }
-    unsigned long long slot = atomicAdd(&A.q->tail, 1ull);
-    st_c(&A.ring[slot % A.ring_size], (slot << 32) | c);
+    ring_push(A, bound, c);
     return true;
 }
 
@@ -667,7 +691,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // read one item ahead so that no item start waits on a global load.
     unsigned long long pnext = 0;
     uint32_t found_seen = 0;
-    uint32_t chunk_left = 0, item_cur = 0;  // persistent form: rest of this lane's chunk
+    // persistent form: the lane's chunk, items left (low byte) | log2 chunk size << 8
+    uint32_t chunk_left = 0, item_cur = 0;
     // The lane's frames stay at depth `hold` on the previous item's prefix
     // (-1: nothing held); `hfail` is the depth of that item's failing prefix
     // move (-1: none).  The next item of the same cube and round undoes back
@@ -722,7 +747,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 }
                 if (QUEUE) {
                     chunk_nodes += nodes;
-                    if (!chunk_left || found_seen) {
+                    if (!(chunk_left & 0xFFu) || found_seen) {
                         chunk_left = 0;
                         end_chunk(A, c, J, chunk_nodes);
                     }
@@ -755,11 +780,18 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     found_seen = 0;
                     bound = J->bound;
                     plen = (uint32_t)A.plen;
-                } else if (chunk_left) {
+                } else if (chunk_left & 0xFFu) {
                     // next item of this lane's chunk: same cube and round, no atomics
+                    const uint32_t n_items = A.pre_n[plen];
+                    item = next_item(item_cur, chunk_left & 0xFFu, chunk_left >> 8, n_items);
                     --chunk_left;
-                    item = ++item_cur;
                     fresh = false;
+                    if (item >= n_items) {  // the rest of the chunk lies past the round's last item
+                        chunk_left = 0;
+                        end_chunk(A, c, J, chunk_nodes);
+                        continue;
+                    }
+                    item_cur = item;
                 } else {
                     if (idle && (--idle)) break;  // back off between empty fetches
                     unsigned long long mask = __ballot(1);
@@ -777,17 +809,25 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                                 continue;
                             if (h >= t) break;
                             unsigned long long e = ld_c(&A.ring[h % A.ring_size]);
-                            if ((e >> 32) != (h & 0xFFFFFFFFull)) break;  // entry still being published
-                            uint32_t ce = (uint32_t)e;
+                            if ((e >> 37) != (h & RING_SEQ_MASK)) break;  // entry still being published
+                            uint32_t ce = (uint32_t)e, eb = (uint32_t)(e >> 32) & 31u;
                             SearchJob* Je = &A.jobs[ce];
                             unsigned long long tk = ld_c(&Je->ticket);
-                            if ((tk & (T_CLOSING | T_RETIRED)) || t_next(tk) >= A.pre_n[t_plen(tk)] ||
-                                ld_c(&Je->found)) {
-                                atomicCAS(&A.q->head, h, h + 1);
+                            if ((tk & (T_CLOSING | T_RETIRED)) || t_bound(tk) != eb ||
+                                t_next(tk) >= A.pre_n[t_plen(tk)] || ld_c(&Je->found)) {
+                                atomicCAS(&A.q->head, h, h + 1);  // stale, finished or fully handed out
                                 continue;
                             }
                             clog = t_chunk(tk);
                             old = atomicAdd(&Je->ticket, (unsigned long long)cnt * (T_INF_ONE + (1ull << clog)));
+                            // Round robin: the cube moves behind the others, so each live cube
+                            // gets about its share of the lanes and its round is searched
+                            // roughly in item order (a final round then stops near where a
+                            // sequential search would).  Only the wave that advances the head
+                            // re-queues the entry, and only while items remain.
+                            if (atomicCAS(&A.q->head, h, h + 1) == h &&
+                                t_next(old) + ((unsigned long long)cnt << clog) < A.pre_n[t_plen(old)])
+                                ring_push(A, eb, ce);
                             cc = ce;
                             break;
                         }
@@ -820,10 +860,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     J = &A.jobs[c];
                     bound = t_bound(old);
                     plen = pl;
+                    item = (item >> clog) * min(ITEM_BLOCK, 1u << clog);  // chunk's first item
                     item_cur = item;
                     fresh = true;  // a chunk may follow a reuse of the slot: start from the root
                     hold = -1;
-                    chunk_left = min(item + (1u << clog), n_items) - item - 1u;
+                    chunk_left = ((1u << clog) - 1u) | (clog << 8);
                     chunk_nodes = 0;
                     if (ld_c(&J->found)) {
                         chunk_left = 0;
@@ -836,7 +877,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 unsigned long long nb;
                 if (QUEUE) {
                     nb = fresh ? A.pre_tab[plen][item] : pnext;
-                    if (chunk_left) pnext = A.pre_tab[plen][item + 1u];  // used by the next item
+                    if (chunk_left & 0xFFu) {  // the next item's prefix, used when it starts
+                        const uint32_t n_items = A.pre_n[plen];
+                        const uint32_t nx = next_item(item, chunk_left & 0xFFu, chunk_left >> 8, n_items);
+                        if (nx < n_items) pnext = A.pre_tab[plen][nx];
+                    }
                     found_seen = ld_c(&J->found);  // looked at when this item ends
                 } else {
                     nb = A.prefixes[item];

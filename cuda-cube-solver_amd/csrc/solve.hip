@@ -272,7 +272,9 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
         }
         const unsigned long long ring_size = 2ull * nl + 1024;
         std::vector<unsigned long long> ring(ring_size, ~0ull);
-        for (int k = 0; k < nl; ++k) ring[k] = ((unsigned long long)k << 32) | (unsigned)k;
+        // ring entries: slot << 37 | round bound << 32 | cube (the kernel's ring_entry)
+        for (int k = 0; k < nl; ++k)
+            ring[k] = ((unsigned long long)k << 37) | ((unsigned long long)lj[k].bound << 32) | (unsigned)k;
         QueueCtl q{};
         q.head = 0;
         q.tail = (unsigned long long)nl;
