@@ -320,10 +320,11 @@ __global__ void __launch_bounds__(256) encode_b3_kernel(const uint32_t* __restri
 
 // ---------------------------------------------------------------------------
 // IDA* search: the reference's step machine (kernels.cu:1447-1801) on wave64.
-// Each outer iteration every lane either performs exactly one table probe
-// (frame k of its current child m) or idles; control work (push, pick the
-// next canonical child, pop, fetch a work item) runs between probes.  The
-// per-lane DFS stack (parent bounds, moves) lives in LDS, lane-major.
+// Each outer iteration every lane either probes its current child m (its next
+// forward and next inverse frame, two table reads in flight) or idles; control
+// work (push, pick the next canonical child, pop, fetch a work item, replay a
+// prefix move) runs between probes, one pass per step in the common cases.
+// The per-lane DFS stack (parent bounds, moves) lives in LDS, lane-major.
 //
 // h and child values are packed 6 x 5 bits in one register.
 
