@@ -660,7 +660,10 @@ __device__ __forceinline__ Frame load_frame_c(const Frame* p) {
     return Frame{ld_c(&w[0]), ld_c(&w[1]), ld_c(&w[2]), ld_c(&w[3]), ld_c(&w[4])};
 }
 
-template <int K, int F, bool QUEUE>
+// QUEUE: the persistent form (cubes and rounds on the device); SERVE: its
+// serving variant (admission from the mailbox, host stop flag), compiled
+// separately so that the batch kernel carries none of that code.
+template <int K, int F, bool QUEUE, bool SERVE>
 __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kernel(SearchArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     CubeLUT& L = *reinterpret_cast<CubeLUT*>(smem);
@@ -799,13 +802,19 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     uint32_t cnt = __popcll(mask), rank = __popcll(mask & lanemask_lt());
                     uint32_t cc = 0xFFFFFFFFu, clog = 0;
                     unsigned long long old = 0;
-                    if (rank == 0) {
+                    // serving: a stop request ends the launch within a few fetches even
+                    // while cubes remain.  The fetch count is kept wave-uniform (each
+                    // fetch takes its leader's count + 1) and the flag, in host memory,
+                    // is read on every 4th fetch.
+                    if (SERVE) atick = first_lane_u32(atick) + 1u;
+                    const bool stopping = SERVE && rank == 0 && (atick & 3u) == 0u && sys_load(A.host_stop);
+                    if (rank == 0 && !stopping) {
                         for (int tries = 0; tries < 32; ++tries) {
                             unsigned long long h = ld_c(&A.q->head), t = ld_c(&A.q->tail);
                             // serving: take new cubes when the ring is empty, and while
                             // fewer than ADMIT_BELOW rounds are queued on every 16th fetch
                             // (the mailbox is in host memory; its reads cross the bus)
-                            if (A.serving && (h >= t || (t - h < ADMIT_BELOW && (++atick & 15u) == 0u)) &&
+                            if (SERVE && (h >= t || (t - h < ADMIT_BELOW && (atick & 15u) == 1u && tries == 0)) &&
                                 admit_one<K, F>(A, L))
                                 continue;
                             if (h >= t) break;
@@ -838,7 +847,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     clog = first_lane_u32(clog);
                     if (cc == 0xFFFFFFFFu) {
                         if (ld_c(&A.q->n_final) >= A.q->n_jobs || ld_c(&A.q->stop) ||
-                            (A.serving && sys_load(A.host_stop))) {
+                            (SERVE && sys_load(A.host_stop))) {
                             done = true;
                         } else {
                             idle = 64;
@@ -901,7 +910,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 hfail = -1;
                 if (fresh) {
                     hp = 0;
-                    if (QUEUE && A.serving) {  // the slot may have held another cube: read past the L2
+                    if (SERVE) {  // the slot may have held another cube: read past the L2
                         static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame_c(&J->frame[j]); });
 #pragma unroll
                         for (int j = 0; j < 6; ++j) hp |= ld_c(&J->h[j]) << (5 * j);
@@ -1068,10 +1077,14 @@ __global__ void __launch_bounds__(256) gather_kernel(const uint32_t* __restrict_
     atomicAdd(sum, (unsigned long long)acc);
 }
 
+template <int K, int F>
+void* search_fn_f(int form) {
+    return form == 2 ? (void*)search_kernel<K, F, true, true>
+         : form == 1 ? (void*)search_kernel<K, F, true, false> : (void*)search_kernel<K, F, false, false>;
+}
 template <int K>
-void* search_fn(int F, bool queue) {
-    if (F == 0) return queue ? (void*)search_kernel<K, 0, true> : (void*)search_kernel<K, 0, false>;
-    return queue ? (void*)search_kernel<K, 1, true> : (void*)search_kernel<K, 1, false>;
+void* search_fn(int F, int form) {
+    return F == 0 ? search_fn_f<K, 0>(form) : search_fn_f<K, 1>(form);
 }
 
 #define CCS_KIND_SWITCH(K, CALL)                  \
@@ -1088,27 +1101,22 @@ void* search_fn(int F, bool queue) {
 
 size_t search_smem_bytes(int block) { return sizeof(CubeLUT) + (size_t)block * (STACK_DEPTH * 4 + STACK_DEPTH); }
 
-hipError_t search_occupancy(int K, int F, bool queue, int block, int* blocks_per_cu) {
+hipError_t search_occupancy(int K, int F, int form, int block, int* blocks_per_cu) {
     void* fn = nullptr;
-#define CALL(KK) fn = search_fn<KK>(F, queue)
+#define CALL(KK) fn = search_fn<KK>(F, form)
     CCS_KIND_SWITCH(K, CALL)
 #undef CALL
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, block, search_smem_bytes(block));
 }
 
-hipError_t launch_search(int K, int F, bool queue, const SearchArgs& args, int grid, int block, hipStream_t s) {
+hipError_t launch_search(int K, int F, int form, const SearchArgs& args, int grid, int block, hipStream_t s) {
     size_t smem = search_smem_bytes(block);
-#define CALL(KK)                                                                                     \
-    if (F == 0) {                                                                                    \
-        if (queue) hipLaunchKernelGGL((search_kernel<KK, 0, true>), dim3(grid), dim3(block), smem, s, args); \
-        else hipLaunchKernelGGL((search_kernel<KK, 0, false>), dim3(grid), dim3(block), smem, s, args);     \
-    } else {                                                                                         \
-        if (queue) hipLaunchKernelGGL((search_kernel<KK, 1, true>), dim3(grid), dim3(block), smem, s, args); \
-        else hipLaunchKernelGGL((search_kernel<KK, 1, false>), dim3(grid), dim3(block), smem, s, args);     \
-    }
+    void* fn = nullptr;
+#define CALL(KK) fn = search_fn<KK>(F, form)
     CCS_KIND_SWITCH(K, CALL)
 #undef CALL
-    return hipGetLastError();
+    void* params[] = {const_cast<SearchArgs*>(&args)};
+    return hipLaunchKernel(fn, dim3(grid), dim3(block), params, smem, s);
 }
 
 hipError_t launch_probe(int K, int F, const CubeLUT* lut, const uint32_t* classidx, const uint8_t* table,

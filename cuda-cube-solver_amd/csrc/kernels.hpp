@@ -17,8 +17,9 @@
 namespace ccs {
 
 size_t search_smem_bytes(int block);
-hipError_t search_occupancy(int K, int F, bool queue, int block, int* blocks_per_cu);
-hipError_t launch_search(int K, int F, bool queue, const SearchArgs& args, int grid, int block, hipStream_t s);
+// form: 0 plain (one round of the listed cubes), 1 persistent batch, 2 persistent serving
+hipError_t search_occupancy(int K, int F, int form, int block, int* blocks_per_cu);
+hipError_t launch_search(int K, int F, int form, const SearchArgs& args, int grid, int block, hipStream_t s);
 hipError_t launch_probe(int K, int F, const CubeLUT* lut, const uint32_t* classidx, const uint8_t* table,
                         const Frame* in, int n, int move, unsigned long long* idx_out, uint32_t* lb_out,
                         Frame* state_out, hipStream_t s);

@@ -60,7 +60,7 @@ unsigned long long to_u64(double x) { return x >= 1.8e19 ? ~0ull : (unsigned lon
 int search_grid(ccs_tables* t, bool queue, int block, int blocks_per_cu, int* grid) {
     int per_cu = blocks_per_cu;
     if (per_cu <= 0) {
-        HIP_TRY(search_occupancy(t->K, t->format, queue, block, &per_cu));
+        HIP_TRY(search_occupancy(t->K, t->format, queue ? 1 : 0, block, &per_cu));
         if (per_cu <= 0) return fail(CCS_EDEVICE, "search kernel does not fit on a CU");
     }
     *grid = per_cu * device_cus();
@@ -215,7 +215,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
                 A.prefixes = t->d_prefix[plen];
                 A.n_prefix = t->prefix_n[plen];
                 A.counter = d_counter;
-                HIP_TRY(launch_search(t->K, t->format, false, A, grid, o.block, t->stream));
+                HIP_TRY(launch_search(t->K, t->format, 0, A, grid, o.block, t->stream));
                 ++launches;
                 off += kv.second.size();
             }
@@ -298,7 +298,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             A.idle_limit = (unsigned long long)std::max(rate_khz0, 1) * 60000ull;  // 60 s
             hipLaunchKernelGGL(read_clock_kernel, dim3(1), dim3(1), 0, t->stream, d_clock);
             HIP_TRY(hipEventRecord(ev0, t->stream));
-            HIP_TRY(launch_search(t->K, t->format, true, A, grid, o.block, t->stream));
+            HIP_TRY(launch_search(t->K, t->format, 1, A, grid, o.block, t->stream));
             ++launches;
             HIP_TRY(hipEventRecord(ev1, t->stream));
             HIP_TRY(hipMemcpyAsync(lj.data(), d_jobs, sizeof(SearchJob) * (size_t)nl, hipMemcpyDeviceToHost, t->stream));

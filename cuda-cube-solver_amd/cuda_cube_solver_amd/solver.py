@@ -150,7 +150,7 @@ class GpuSolver:
 
     def __init__(self, tables: Path = Path("tables"), kind: str = "d4t", table_format: str = "block",
                  mode: str = "plain", prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0,
-                 poll_interval: int = 64, chunk_nodes: int = 0, table: Tables | None = None):
+                 poll_interval: int = 64, chunk_nodes: int = 0, table: Tables | None = None, max_bound: int = 0):
         if table is None:
             tables = Path(tables)
             if tables.suffix == ".bin":
@@ -165,6 +165,7 @@ class GpuSolver:
         self.blocks_per_cu = blocks_per_cu
         self.poll_interval = poll_interval
         self.chunk_nodes = chunk_nodes
+        self.max_bound = max_bound  # 0: no limit below the longest optimal solution (20)
         self.last_stats: _lib.SolveStats | None = None
 
     def _opts(self, mode: str | None = None) -> _lib.SolveOpts:
@@ -174,7 +175,7 @@ class GpuSolver:
         o.deeper_at[0], o.deeper_at[1] = self.deeper_prefix_at
         o.block = 0
         o.blocks_per_cu = self.blocks_per_cu
-        o.max_bound = 0
+        o.max_bound = self.max_bound
         o.poll_interval = self.poll_interval
         o.chunk_nodes = self.chunk_nodes
         return o

@@ -192,3 +192,41 @@ def test_serving_results_equal_alone_and_in_the_mix(td4t):
         r = got[i]
         assert r.length == len(alone[i].moves)
         assert apply_moves(c, [MOVES_HTM[m] for m in r.moves]) == CubieState()
+
+
+def test_max_bound_below_the_optimum_is_a_clean_error(td4t, reference_results):
+    """A bound limit below a cube's optimal length ends both forms with the
+    reference's assertion (not solved), and the solver keeps working."""
+    ref = reference_results["random1000"]
+    i = next(k for k, r in enumerate(ref["results"]) if r["len"] == 17)
+    cubes = _scrambled([ref["scrambles"][i]] + ref["scrambles"][300:303])
+    for mode in ("plain", "batched"):
+        with pytest.raises(AssertionError, match="not solved within max_bound"):
+            GpuSolver(table=td4t, mode=mode, max_bound=16).solve(cubes)
+    sols = GpuSolver(table=td4t, mode="batched").solve(cubes)
+    assert [len(s.moves) for s in sols] == [17] + [r["len"] for r in ref["results"][300:303]]
+
+
+def test_server_stops_promptly_while_busy(td4t):
+    """Stopping a server with hard cubes still in flight returns within
+    seconds (pending and unfinished cubes are dropped); a new server on the
+    same table then serves normally."""
+    import time
+
+    from cuda_cube_solver_amd.server import Server
+
+    busy = states.flip_heavy_states(32, 61) + states.uniform_states(96, 62)
+    srv = Server(td4t)
+    for i, c in enumerate(busy):
+        srv.submit(c, tag=i)
+    time.sleep(0.3)
+    t = time.perf_counter()
+    srv.stop()
+    assert time.perf_counter() - t < 10.0
+    probe = states.uniform_states(4, 63)
+    alone = GpuSolver(table=td4t, mode="batched").solve(probe)
+    with Server(td4t) as again:
+        for i, c in enumerate(probe):
+            again.submit(c, tag=i)
+        got = again.collect(len(probe), timeout_s=120)
+    assert [got[i].length for i in range(len(probe))] == [len(s.moves) for s in alone]
