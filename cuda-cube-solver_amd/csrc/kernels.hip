@@ -400,13 +400,11 @@ __device__ __forceinline__ Frame load_frame(const Frame* p) {
     const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
     return Frame{w[0], w[1], w[2], w[3], w[4]};
 }
-__device__ __forceinline__ void frame_do(const CubeLUT& L, Frame& f, int j, int m) {
-    if (j < 3) right_mul(f, L.rr[0][j][m]);
-    else left_mul(f, L.lr[0][j - 3][m]);
-}
-__device__ __forceinline__ void frame_undo(const CubeLUT& L, Frame& f, int j, int m) {
-    if (j < 3) right_mul(f, L.rr[1][j][m]);
-    else left_mul(f, L.lr[1][j - 3][m]);
+// Frame j after move m (sel 0) or after undoing move m (sel 1): one code path
+// for both, so that lanes pushing and lanes popping share its instructions.
+__device__ __forceinline__ void frame_move(const CubeLUT& L, Frame& f, int j, uint32_t sel, int m) {
+    if (j < 3) right_mul(f, L.rr[sel][j][m]);
+    else left_mul(f, L.lr[sel][j - 3][m]);
 }
 
 // Ticket of the persistent form: one 64-bit word per cube packs the round
@@ -707,24 +705,34 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
 
     for (;;) {
         bool eval = false;
+        // The pass's frame update, applied after the pass for every lane at once
+        // (push: move mv; pop: undo move mv): -1 none.  `leaf`: the push reached
+        // the bound, so the frames are checked for the solved cube after it.
+        int mv = -1;
+        uint32_t msel = 0;
+        bool leaf = false;
         while (!done) {
             // One pass resolves the lane's evaluated child, then finishes, fetches,
             // undoes or backs up as needed and picks the next child, so that the
             // common sequences (push then pick, discard then pick, pop then pick)
-            // cost the wave a single pass.
+            // cost the wave a single pass.  Once a frame update is pending, only
+            // work that does not read the frames (a pick, the next prefix move)
+            // may follow in the same pass.
+            if (mv >= 0) break;
             if (m >= 0) {
                 if (!cdone) {  // frames of child m still to probe
                     eval = true;
                     break;
                 }
                 if (!pruned && !all_tight(cp, bound - (uint32_t)g - 1u)) {  // survived: push
-                    static_for<0, 6>([&](auto j) { frame_do(L, f.template at<j>(), j, m); });
+                    mv = m;
+                    msel = 0;
                     hst[g * nt] = hp;
                     pth[g * nt] = (uint8_t)m;
                     hp = cp;
                     ++g;
                     if ((uint32_t)g == bound) {
-                        if (is_solved(f.f0)) report_solution(A, J, QUEUE, g, pth, nt);
+                        leaf = true;
                         todo = 0;
                     } else {
                         todo = L.amask[m + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
@@ -736,6 +744,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     phase = 4;
                 }
                 m = -1;
+                if (mv >= 0 && !(phase == 2 || (phase == 3 && todo && !found_seen))) break;
             }
             bool pop = phase == 1;
             if (phase == 3 && todo == 0) {  // node exhausted: the item is complete, or back up
@@ -934,7 +943,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             if (pop) {
                 --g;
                 const int pm = pth[g * nt];
-                static_for<0, 6>([&](auto j) { frame_undo(L, f.template at<j>(), j, pm); });
+                mv = pm;
+                msel = 1;
                 hp = hst[g * nt];
                 const int last = g > 0 ? (int)pth[(g - 1) * nt] : -1;
                 todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g)) & ~((2u << pm) - 1u);
@@ -982,9 +992,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             }
         }
         if (__all(done)) break;
-        if (QUEUE && !__any(eval)) {  // the whole wave is waiting for work
+        if (QUEUE && !__any(eval || mv >= 0)) {  // the whole wave is waiting for work
             __builtin_amdgcn_s_sleep(2);
             continue;
+        }
+        if (mv >= 0) {
+            static_for<0, 6>([&](auto j) { frame_move(L, f.template at<j>(), j, msel, mv); });
+            if (leaf && is_solved(f.f0)) report_solution(A, J, QUEUE, g, pth, nt);
         }
         if (eval) {
             // ---- one step of child m: its next forward and next inverse frame,
