@@ -45,6 +45,40 @@ def pct(xs, q):
     return xs[k]
 
 
+def build_summary(lines: list[str], counts: list[int], total_s: float) -> dict:
+    """The table build from the library's log: one BFS layer per depth
+    ("depth d fwd|bwd found n (t s)"), then the block and base-3 encoders
+    ("encoded <fmt> table, <bytes> bytes (t s)").  A top-down layer expands the
+    previous depth's entries, 18 moves each; a bottom-up layer tests every
+    still-unvisited entry until one of its 18 neighbours sits at the previous
+    depth."""
+    import re
+
+    layers, enc = [], {}
+    remaining = sum(counts) - counts[0]
+    for s in lines:
+        m = re.match(r"depth\s+(\d+) (fwd|bwd) found\s+(\d+) \(\s*([\d.]+)s\)", s)
+        if m:
+            d, how, found, sec = int(m[1]), m[2], int(m[3]), float(m[4])
+            row = {"depth": d, "direction": "top-down" if how == "fwd" else "bottom-up", "found": found,
+                   "seconds": sec}
+            if how == "fwd":
+                row["expanded"] = counts[d - 1]
+                row["neighbours_per_s"] = round(18 * counts[d - 1] / sec) if sec > 0 else None
+            else:
+                row["tested"] = remaining
+            layers.append(row)
+            remaining -= found
+        m = re.match(r"encoded (\w+) table, (\d+) bytes \(\s*([\d.]+)s\)", s)
+        if m:
+            enc[m[1]] = {"bytes": int(m[2]), "seconds": float(m[3]),
+                         "written_gb_s": round(int(m[2]) / float(m[3]) / 1e9, 1) if float(m[3]) > 0 else None}
+    bfs_s = sum(r["seconds"] for r in layers)
+    return {"entries": sum(counts), "depth_counts": counts, "seconds": round(total_s, 2),
+            "bfs_seconds": round(bfs_s, 2), "entries_per_s": round(sum(counts) / bfs_s) if bfs_s else None,
+            "layers": layers, "encode": enc}
+
+
 def gather_ceiling(tb) -> dict:
     """Random-probe ceiling of the memory system on this table: 2^32 reads of a
     random 32-byte block's minimum word plus one offset word (the search's
@@ -118,8 +152,11 @@ def main() -> None:
     dev = _lib.device_info()
     log(f"device {dev.name.decode()} {dev.arch.decode()} {dev.compute_units} CUs")
     t = time.perf_counter()
-    tb, counts = Tables.build(args.table, "block", log=log)
-    log(f"built {args.table} in {time.perf_counter() - t:.1f}s, depth counts {counts}")
+    build_log: list[str] = []
+    # block and base-3 tables from scratch; searches use the block table
+    tb, counts = Tables.build(args.table, "both", log=lambda s: (build_log.append(s), log(s)))
+    build_s = time.perf_counter() - t
+    log(f"built {args.table} in {build_s:.1f}s, depth counts {counts}")
     cubes = states.uniform_states(args.cubes, args.seed)
     knobs = {"chunk_nodes": args.chunk_nodes, "blocks_per_cu": args.blocks_per_cu, "prefix_len": args.prefix_len,
              "deeper_prefix_at": (5e7, 3e9)}
@@ -205,6 +242,7 @@ def main() -> None:
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(probes / args.steps * (FETCHED_BYTES_PER_PROBE + WRITTEN_BYTES_PER_PROBE))},
     }
+    line["table_build"] = build_summary(build_log, counts, build_s)
     if not args.no_gather:
         line["probe_ceiling"] = gather_ceiling(tb)
     if not args.no_serve:

@@ -211,7 +211,7 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
         HIP_TRY(hipMemcpyAsync(&found, slot, 8, hipMemcpyDeviceToHost, t->stream));
         HIP_TRY(hipStreamSynchronize(t->stream));
         char line[160];
-        snprintf(line, sizeof line, "depth %2u %s found %14llu (%7.2fs)", depth + 1, fwd ? "fwd" : "bwd", found,
+        snprintf(line, sizeof line, "depth %2u %s found %14llu (%7.3fs)", depth + 1, fwd ? "fwd" : "bwd", found,
                  (now_ms() - t0) / 1e3);
         logf(line);
         if (found == 0) {
@@ -246,11 +246,17 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
             return fail(CCS_ENOMEM, "solving table allocation");
         }
         t->table_bytes[fmt] = nb * 32;
+        const double te = now_ms();
         const uint64_t CH = 1ull << 26;  // blocks per launch
         for (uint64_t b0 = 0; b0 < nb; b0 += CH)
             HIP_TRY(launch_encode(fmt, tbl, reinterpret_cast<const uint32_t*>(t->d_table[0]), n_entries, b0,
                                   std::min(CH, nb - b0), reinterpret_cast<uint32_t*>(t->d_table[fmt]) + b0 * 8,
                                   d_bad, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        char line[160];
+        snprintf(line, sizeof line, "encoded %s table, %llu bytes (%7.3fs)", fmt ? "b3" : "block",
+                 (unsigned long long)(nb * 32), (now_ms() - te) / 1e3);
+        logf(line);
     }
     unsigned int bad = 0;
     HIP_TRY(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, t->stream));

@@ -5,6 +5,7 @@
 
 #include <type_traits>
 
+#include "encode.hpp"
 #include "kernels.hpp"
 
 namespace ccs {
@@ -96,27 +97,36 @@ __device__ __forceinline__ uint32_t nib_get(const uint32_t* tbl, uint64_t idx) {
     return (tbl[idx >> 3] >> ((uint32_t)(idx & 7u) * 4u)) & 15u;
 }
 
+// Claim entry idx at `depth` if it is unvisited; 1 if this call set it.
+__device__ __forceinline__ uint32_t claim_entry(uint32_t* tbl, uint64_t idx, uint32_t depth) {
+    uint32_t* wp = tbl + (idx >> 3);
+    const uint32_t sh = (uint32_t)(idx & 7u) * 4u;
+    if (((*wp >> sh) & 15u) != 15u) return 0;
+    const uint32_t old = atomicAnd(wp, ~((15u ^ depth) << sh));
+    return ((old >> sh) & 15u) == 15u;
+}
+
 // Claim the canonical index of nb and its stabiliser images at `depth`
 // (kernels.cu:507-528 claims the same set).  Returns the number newly set.
 template <int K>
 __device__ uint32_t claim_images(const CubeLUT& L, uint32_t* tbl, const uint32_t* __restrict__ classidx,
                                  const uint16_t* __restrict__ class_stab, const Frame& nb, uint32_t depth) {
-    uint32_t packed = classidx[corner_raw(L, nb)];
-    uint32_t cls = packed >> 4, sym = packed & 15u;
-    uint32_t stab = class_stab[cls];
-    uint64_t base = (uint64_t)cls * KindT<K>::N_EDGE;
+    const uint32_t packed = classidx[corner_raw(L, nb)];
+    const uint32_t cls = packed >> 4, sym = packed & 15u;
+    const uint64_t base = (uint64_t)cls * KindT<K>::N_EDGE;
     uint32_t claimed = 0;
-    while (stab) {
-        uint32_t t = ctz(stab);
-        stab &= stab - 1;
-        uint64_t idx = base + edge_coord<K>(L, nb, L.s_mul[t][sym]);
-        uint32_t* wp = tbl + (idx >> 3);
-        uint32_t sh = (uint32_t)(idx & 7u) * 4u;
-        if (((*wp >> sh) & 15u) != 15u) continue;
-        uint32_t old = atomicAnd(wp, ~((15u ^ depth) << sh));
-        claimed += ((old >> sh) & 15u) == 15u;
-    }
+    for (uint32_t stab = class_stab[cls]; stab; stab &= stab - 1u)
+        claimed += claim_entry(tbl, base + edge_coord<K>(L, nb, L.s_mul[ctz(stab)][sym]), depth);
     return claimed;
+}
+
+// Memory order among the lanes of one wave for its LDS lists (the LDS runs a
+// wave's instructions in order; the fences keep the compiler from moving the
+// accesses across the barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int K>
@@ -128,18 +138,129 @@ __global__ void bfs_seed_kernel(const CubeLUT* __restrict__ gL, uint32_t* tbl, c
     atomicAdd(counter, (unsigned long long)claim_images<K>(L, tbl, classidx, class_stab, solved_frame(), 0u));
 }
 
-// One BFS layer.  FWD: expand entries at `depth` and claim their neighbours
-// at depth + 1 (top-down).  !FWD: an unvisited entry with a neighbour at
-// `depth` is at depth + 1 (bottom-up).  Each wave loads 64 consecutive words,
-// compacts the matching entries into an LDS list and shares them across its
-// 64 lanes, so a word with 8 hits costs the same as 8 words with one.
-template <int K, bool FWD>
-__global__ void __launch_bounds__(BFS_WG) bfs_layer_kernel(const CubeLUT* __restrict__ gL, uint32_t* tbl,
-                                                          const uint32_t* __restrict__ classidx,
-                                                          const uint32_t* __restrict__ class_rep,
-                                                          const uint16_t* __restrict__ class_stab,
-                                                          unsigned long long n_words, unsigned long long n_entries,
-                                                          uint32_t depth, unsigned long long* counter) {
+// BFS layers.  Each wave scans 64 consecutive words (512 entries) at a time,
+// grid-strided, and compacts the entries to process into an LDS list, so that
+// lanes work on entries rather than on words.
+
+// Top-down layer: expand the entries at `depth` and claim their neighbours at
+// depth + 1.  The list carries over from chunk to chunk and is worked off 64
+// entries at a time, so sparse layers keep every lane busy.  A neighbour whose
+// corner class has a nontrivial stabiliser has several table entries
+// (kernels.cu:507-528 claims them all); its first is claimed at once and the
+// rest are queued in LDS and claimed 64 neighbours at a time, so the few lanes
+// with extra images do not hold up the wave at every move.
+constexpr uint32_t FWD_LIST = 576;   // < 64 carried plus one chunk's 512
+constexpr uint32_t FWD_QUEUE = 128;  // < 64 carried plus one move's 64
+constexpr uint32_t FWD_TASK = 7;     // queued neighbour: frame (5 words), class, sym | images << 8
+constexpr uint32_t FWD_WG = 512;     // eight waves share the LUT copy: 4 waves per SIMD fit the LDS
+
+template <int K>
+__global__ void __launch_bounds__(FWD_WG) bfs_forward_kernel(const CubeLUT* __restrict__ gL, uint32_t* tbl,
+                                                            const uint32_t* __restrict__ classidx,
+                                                            const uint32_t* __restrict__ class_rep,
+                                                            const uint16_t* __restrict__ class_stab,
+                                                            unsigned long long n_words, unsigned long long n_entries,
+                                                            uint32_t depth, unsigned long long* counter) {
+    __shared__ CubeLUT L;
+    __shared__ uint32_t lst[FWD_WG / 64][FWD_LIST];
+    __shared__ uint32_t xq[FWD_WG / 64][FWD_QUEUE * FWD_TASK];
+    __shared__ uint32_t cnt[FWD_WG / 64], xcnt[FWD_WG / 64];
+    load_lut(&L, gL);
+    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t next = depth + 1u;
+    const uint32_t pat = depth * 0x11111111u;
+    unsigned long long found = 0;
+    const unsigned long long n_chunks = (n_words + 63) / 64;
+    const unsigned long long stride = (unsigned long long)gridDim.x * (FWD_WG / 64);
+    const unsigned long long c0 = (unsigned long long)blockIdx.x * (FWD_WG / 64) + wave;
+    if (lane == 0) cnt[wave] = xcnt[wave] = 0;
+    wave_sync();
+    // claim the remaining images of up to 64 queued neighbours, from the top of the queue
+    auto drain_queue = [&](uint32_t take) {
+        const uint32_t q = xcnt[wave];
+        if (lane < take) {
+            const uint32_t* t = &xq[wave][(q - take + lane) * FWD_TASK];
+            const Frame nb{t[0], t[1], t[2], t[3], t[4]};
+            const uint64_t base = (uint64_t)t[5] * KindT<K>::N_EDGE;
+            const uint32_t sym = t[6] & 15u;
+            for (uint32_t st = t[6] >> 8; st; st &= st - 1u)
+                found += claim_entry(tbl, base + edge_coord<K>(L, nb, L.s_mul[ctz(st)][sym]), next);
+        }
+        wave_sync();
+        if (lane == 0) xcnt[wave] = q - take;
+        wave_sync();
+    };
+    for (uint32_t k = 0;; ++k) {
+        const unsigned long long chunk = c0 + (unsigned long long)k * stride;
+        const bool have = chunk < n_chunks;  // uniform across the wave
+        if (have) {
+            const unsigned long long w = chunk * 64 + lane;
+            uint32_t hits = 0;
+            if (w < n_words) {
+                const uint32_t x = tbl[w] ^ pat;
+                hits = ~(x | (x >> 1) | (x >> 2) | (x >> 3)) & 0x11111111u;
+                const unsigned long long first = w * 8u;
+                if (first + 8u > n_entries) hits &= (1u << (4u * (uint32_t)(n_entries - first))) - 1u;
+            }
+            // entry = (chunk number k within this wave, word lane, nibble); k < 2^23
+            uint32_t off = atomicAdd(&cnt[wave], popc(hits));
+            for (uint32_t h = hits; h; h &= h - 1) lst[wave][off++] = (k << 9) | (lane << 3) | (ctz(h) >> 2);
+        }
+        wave_sync();
+        uint32_t total = cnt[wave];
+        const bool last = !have || chunk + stride >= n_chunks;
+        while (total >= 64u || (last && total > 0u)) {
+            const uint32_t take = min(total, 64u);
+            const bool act = lane < take;
+            Frame s{};
+            if (act) {
+                const uint32_t e = lst[wave][total - take + lane];
+                const unsigned long long ch = c0 + (unsigned long long)(e >> 9) * stride;
+                s = unrank<K>(L, class_rep, (ch * 64u + ((e >> 3) & 63u)) * 8u + (e & 7u));
+            }
+#pragma unroll 1
+            for (int m = 0; m < 18; ++m) {
+                if (act) {
+                    Frame nb = s;
+                    apply_move(L, nb, m);
+                    const uint32_t packed = classidx[corner_raw(L, nb)];
+                    const uint32_t cls = packed >> 4, sym = packed & 15u;
+                    const uint32_t stab = class_stab[cls];
+                    found += claim_entry(tbl, (uint64_t)cls * KindT<K>::N_EDGE +
+                                                  edge_coord<K>(L, nb, L.s_mul[ctz(stab)][sym]), next);
+                    const uint32_t rest = stab & (stab - 1u);
+                    if (rest) {
+                        uint32_t* t = &xq[wave][atomicAdd(&xcnt[wave], 1u) * FWD_TASK];
+                        t[0] = nb.c0, t[1] = nb.c1, t[2] = nb.e0, t[3] = nb.e1, t[4] = nb.e2;
+                        t[5] = cls, t[6] = sym | (rest << 8);
+                    }
+                }
+                wave_sync();
+                if (xcnt[wave] >= 64u) drain_queue(64u);
+            }
+            total -= take;
+        }
+        if (lane == 0) cnt[wave] = total;
+        wave_sync();
+        if (last) break;
+    }
+    drain_queue(xcnt[wave]);
+    // one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) found += __shfl_xor(found, o);
+    if (lane == 0 && found) atomicAdd(counter, found);
+}
+
+// Bottom-up layer: an unvisited entry with a neighbour at `depth` is at
+// depth + 1.  A lane stops at an entry's first such neighbour, so entries take
+// 1 to 18 moves; lanes that finish take the next entry of the chunk's list
+// once a quarter of the wave is idle, instead of every lane waiting for the
+// wave's slowest entry.  Each lane then writes back its word of the chunk.
+template <int K>
+__global__ void __launch_bounds__(BFS_WG) bfs_backward_kernel(const CubeLUT* __restrict__ gL, uint32_t* tbl,
+                                                             const uint32_t* __restrict__ classidx,
+                                                             const uint32_t* __restrict__ class_rep,
+                                                             unsigned long long n_words, unsigned long long n_entries,
+                                                             uint32_t depth, unsigned long long* counter) {
     __shared__ CubeLUT L;
     __shared__ uint16_t lst[BFS_WG / 64][512];
     __shared__ uint32_t cnt[BFS_WG / 64];
@@ -147,7 +268,6 @@ __global__ void __launch_bounds__(BFS_WG) bfs_layer_kernel(const CubeLUT* __rest
     load_lut(&L, gL);
     const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint32_t next = depth + 1u;
-    const uint32_t pat = depth * 0x11111111u;
     unsigned long long found = 0;
     const unsigned long long n_chunks = (n_words + 63) / 64;
     const unsigned long long stride = (unsigned long long)gridDim.x * (BFS_WG / 64);
@@ -157,63 +277,63 @@ __global__ void __launch_bounds__(BFS_WG) bfs_layer_kernel(const CubeLUT* __rest
         uint32_t word = 0, hits = 0;
         if (w < n_words) {
             word = tbl[w];
-            if (FWD) {
-                uint32_t x = word ^ pat;
-                hits = ~(x | (x >> 1) | (x >> 2) | (x >> 3)) & 0x11111111u;
-            } else {
-                hits = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;
-            }
-            unsigned long long first = w * 8u;
-            if (first + 8u > n_entries) {
-                uint32_t valid = (uint32_t)(n_entries - first);  // 1..7
-                hits &= (1u << (4u * valid)) - 1u;
-            }
+            hits = word & (word >> 1) & (word >> 2) & (word >> 3) & 0x11111111u;  // unvisited (15)
+            const unsigned long long first = w * 8u;
+            if (first + 8u > n_entries) hits &= (1u << (4u * (uint32_t)(n_entries - first))) - 1u;
         }
         if (lane == 0) cnt[wave] = 0;
         clr[wave][lane] = 0;
-        __builtin_amdgcn_wave_barrier();
+        wave_sync();
         uint32_t off = atomicAdd(&cnt[wave], popc(hits));
         for (uint32_t h = hits; h; h &= h - 1) lst[wave][off++] = (uint16_t)((lane << 3) | (ctz(h) >> 2));
-        __builtin_amdgcn_wave_barrier();
+        wave_sync();
         const uint32_t total = cnt[wave];
-        for (uint32_t i = lane; i < total; i += 64) {
-            uint32_t e = lst[wave][i];
-            uint64_t idx = (chunk * 64 + (e >> 3)) * 8u + (e & 7u);
-            Frame s = unrank<K>(L, class_rep, idx);
-            if (FWD) {
-#pragma unroll 1
-                for (int m = 0; m < 18; ++m) {
-                    Frame nb = s;
-                    apply_move(L, nb, m);
-                    found += claim_images<K>(L, tbl, classidx, class_stab, nb, next);
-                }
-            } else {
-#pragma unroll 1
-                for (int m = 0; m < 18; ++m) {
+        uint32_t handed = 0, e = 0;
+        int m = 0;
+        bool act = false;
+        Frame s{};
+        for (;;) {
+            const bool need = !act;
+            const unsigned long long nm = __ballot(need);
+            const uint32_t pos = handed + __popcll(nm & lanemask_lt());
+            if (need && pos < total) {
+                e = lst[wave][pos];
+                s = unrank<K>(L, class_rep, (chunk * 64 + (e >> 3)) * 8u + (e & 7u));
+                m = 0;
+                act = true;
+            }
+            handed = min(total, handed + (uint32_t)__popcll(nm));
+            if (!__any(act)) break;
+            const uint32_t keep = handed < total ? 48u : 0u;  // refill below this many busy lanes
+            do {
+                if (act) {
                     Frame nb = s;
                     apply_move(L, nb, m);
                     if (nib_get(tbl, table_index<K>(L, classidx, nb)) == depth) {
                         atomicOr(&clr[wave][e >> 3], (15u ^ next) << ((e & 7u) * 4u));
                         ++found;
-                        break;
+                        act = false;
+                    } else if (++m == 18) {
+                        act = false;
                     }
                 }
-            }
+            } while ((uint32_t)__popcll(__ballot(act)) > keep);
         }
-        __builtin_amdgcn_wave_barrier();
-        if (!FWD && w < n_words) {
-            uint32_t c = clr[wave][lane];
+        wave_sync();
+        if (w < n_words) {
+            const uint32_t c = clr[wave][lane];
             if (c) tbl[w] = word & ~c;  // this lane owns word w during a bottom-up layer
         }
-        __builtin_amdgcn_wave_barrier();
+        wave_sync();
     }
-    // one atomic per wave
     for (int o = 32; o > 0; o >>= 1) found += __shfl_xor(found, o);
     if (lane == 0 && found) atomicAdd(counter, found);
 }
 
 // Histogram of the nibble values over the valid entries (the reference's
-// count_values self-check, kernels.cu:614-624).
+// count_values self-check, kernels.cu:614-624).  Per word and value k, the
+// nibbles equal to k are marked by a zero test on word ^ (k * 0x11111111) and
+// counted with one popcount.
 __global__ void __launch_bounds__(256) histogram_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_words,
                                                         unsigned long long n_entries,
                                                         unsigned long long* __restrict__ counts) {
@@ -222,13 +342,13 @@ __global__ void __launch_bounds__(256) histogram_kernel(const uint32_t* __restri
     for (int v = 0; v < 16; ++v) local[v] = 0;
     for (unsigned long long w = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; w < n_words;
          w += (unsigned long long)gridDim.x * blockDim.x) {
-        uint32_t word = tbl[w];
-        uint32_t valid = 8;
-        if (w * 8u + 8u > n_entries) valid = (uint32_t)(n_entries - w * 8u);
-        for (uint32_t j = 0; j < valid; ++j) {
-            uint32_t v = (word >> (4 * j)) & 15u;
+        const uint32_t word = tbl[w];
+        uint32_t keep = 0x11111111u;  // one bit per valid nibble
+        if (w * 8u + 8u > n_entries) keep &= (1u << (4u * (uint32_t)(n_entries - w * 8u))) - 1u;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) local[k] += v == (uint32_t)k;
+        for (uint32_t k = 0; k < 16; ++k) {
+            const uint32_t z = word ^ (k * 0x11111111u);
+            local[k] += popc(~(z | (z >> 1) | (z >> 2) | (z >> 3)) & keep);
         }
     }
 #pragma unroll
@@ -239,15 +359,35 @@ __global__ void __launch_bounds__(256) histogram_kernel(const uint32_t* __restri
     }
 }
 
-// Block format (kernels.cu:396-406, 485-503 define it): per 124 entries, u8
-// minimum then 2-bit offsets min(v - min, 3) at bit 8 + 2j.
-__global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_entries,
-                                                           unsigned long long b0, unsigned long long nb,
-                                                           uint32_t* __restrict__ out, unsigned int* __restrict__ bad) {
-    unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    unsigned long long first = (b0 + i) * 124u;
-    uint32_t n = (uint32_t)min(124ull, n_entries - first);
+// Encoders.  One wave encodes 64 consecutive output blocks.  The wave first
+// copies its span of the build table to LDS with 16-byte loads (the span of 64
+// blocks starts on a 16-byte boundary because the host launches from block
+// numbers that are multiples of 64), then each lane encodes one block from
+// registers (encode.hpp).  The table's last block, if partial, takes the
+// per-entry path below.
+
+// Copy words [W, W + 256 * R) of tbl (zero past n_words) to LDS, 16 bytes a lane.
+template <int R>
+__device__ __forceinline__ void stage_span(const uint32_t* __restrict__ tbl, unsigned long long n_words,
+                                           unsigned long long W, uint32_t* st, uint32_t lane) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const unsigned long long wi = W + (unsigned long long)(r * 64 + (int)lane) * 4u;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (wi + 3u < n_words) {
+            v = *reinterpret_cast<const uint4*>(tbl + wi);
+        } else {
+            if (wi < n_words) v.x = tbl[wi];
+            if (wi + 1u < n_words) v.y = tbl[wi + 1u];
+            if (wi + 2u < n_words) v.z = tbl[wi + 2u];
+        }
+        reinterpret_cast<uint4*>(st)[r * 64 + (int)lane] = v;
+    }
+}
+
+// Block format, one block of fewer than 124 entries (the table's last).
+__device__ void encode_block_partial(const uint32_t* __restrict__ tbl, unsigned long long first, uint32_t n,
+                                     uint32_t* __restrict__ out, unsigned int* __restrict__ bad) {
     uint32_t m = 15;
     for (uint32_t j = 0; j < n; ++j) m = min(m, nib_get(tbl, first + j));
     uint32_t w[8] = {m, 0, 0, 0, 0, 0, 0, 0};
@@ -259,63 +399,74 @@ __global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __res
         w[pos >> 5] |= min(v - m, 3u) << (pos & 31u);
     }
     if (unset) atomicOr(bad, 1u);
-    uint4* o = reinterpret_cast<uint4*>(out + i * 8u);
-    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = w[k];
 }
 
-// The value the base-3 encoder sees for entry x (kernels.cu:1825-1835): the
-// block-table bound, and when its 2-bit offset saturates, the first value at
-// or above min + 3 with the entry's depth mod 3.  That equals the depth only
-// up to min + 5, so it is recomputed here from the exact depth and the
-// entry's 124-block minimum to give the same bytes.
-__device__ __forceinline__ uint32_t b3_source_value(const uint32_t* __restrict__ tbl,
-                                                    const uint32_t* __restrict__ blk, unsigned long long x) {
-    uint32_t d = nib_get(tbl, x);
-    uint32_t mn = blk[(x / 124u) * 8u] & 0xFFu;
-    return d - mn < 3u ? d : mn + 3u + (d - mn - 3u) % 3u;
+// Block format (kernels.cu:396-406, 485-503 define it): per 124 entries, u8
+// minimum then 2-bit offsets min(v - min, 3) at bit 8 + 2j.  Blocks b0 .. b0 + nb - 1.
+__global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_words,
+                                                           unsigned long long n_entries, unsigned long long b0,
+                                                           unsigned long long nb, uint32_t* __restrict__ out,
+                                                           unsigned int* __restrict__ bad) {
+    __shared__ uint32_t stage[4][1024];  // 64 blocks x 15.5 words, plus the word after
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const unsigned long long i0 = ((unsigned long long)blockIdx.x * 4u + wave) * 64u;  // wave's first block - b0
+    const unsigned long long gb0 = b0 + i0;
+    if (i0 < nb) stage_span<4>(tbl, n_words, gb0 / 2u * 31u, stage[wave], lane);  // word (gb0 * 124) / 8
+    __syncthreads();
+    const unsigned long long i = i0 + lane;
+    if (i >= nb) return;
+    const unsigned long long first = (gb0 + lane) * BLK_ENTRIES;
+    uint32_t* o = out + i * 8u;
+    if (n_entries - first < BLK_ENTRIES) {
+        encode_block_partial(tbl, first, (uint32_t)(n_entries - first), o, bad);
+        return;
+    }
+    const uint32_t* st = stage[wave] + ((lane * 31u) >> 1);  // block's first word within the span
+    const uint32_t sh = (lane & 1u) * 16u;                   // its first nibble: 0 or 4
+    uint32_t a[16], w[8];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = funnel(st[k + 1], st[k], sh);
+    if (encode_block_words(a, w)) atomicOr(bad, 1u);
+    reinterpret_cast<uint4*>(o)[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    reinterpret_cast<uint4*>(o)[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
 // Base-3 format (kernels.cu:1839-1869 define it): per 155 entries, header
 // m | c << 6 with the cut c in 0..3 maximising the sum of encoded bounds
 // (lowest c on ties), then 31 bytes of five digits, little-endian base 3.
-__global__ void __launch_bounds__(256) encode_b3_kernel(const uint32_t* __restrict__ tbl,
+__global__ void __launch_bounds__(256) encode_b3_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_words,
                                                         const uint32_t* __restrict__ blk, unsigned long long n_entries,
                                                         unsigned long long b0, unsigned long long nb,
                                                         uint32_t* __restrict__ out) {
-    unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+    __shared__ uint32_t stage[4][1280];  // 64 blocks x 19.375 words, plus the word after
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const unsigned long long i0 = ((unsigned long long)blockIdx.x * 4u + wave) * 64u;
+    const unsigned long long gb0 = b0 + i0;
+    if (i0 < nb) stage_span<5>(tbl, n_words, gb0 / 8u * 155u, stage[wave], lane);  // word (gb0 * 155) / 8
+    __syncthreads();
+    const unsigned long long i = i0 + lane;
     if (i >= nb) return;
-    unsigned long long first = (b0 + i) * 155u;
-    uint32_t n = (uint32_t)min(155ull, n_entries - first);
-    uint32_t m = 255;
-    for (uint32_t j = 0; j < n; ++j) m = min(m, b3_source_value(tbl, blk, first + j));
-    uint32_t score[4] = {0, 0, 0, 0};
-    for (uint32_t j = 0; j < n; ++j) {
-        uint32_t v = b3_source_value(tbl, blk, first + j);
-#pragma unroll
-        for (uint32_t c = 0; c < 4; ++c) score[c] += v <= m + c ? m : (v == m + c + 1 ? m + c + 1 : m + c + 2);
+    const unsigned long long first = (gb0 + lane) * B3_ENTRIES;
+    const uint32_t n = (uint32_t)min((unsigned long long)B3_ENTRIES, n_entries - first);
+    uint32_t* o = out + i * 8u;
+    // the block's entries meet at most three 124-entry blocks of the block table
+    const unsigned long long bl = first / BLK_ENTRIES;
+    const uint32_t j1 = (uint32_t)((bl + 1u) * BLK_ENTRIES - first), j2 = j1 + BLK_ENTRIES;
+    const uint32_t mn0 = blk[bl * 8u] & 0xFFu;
+    const uint32_t mn1 = j1 < n ? blk[(bl + 1u) * 8u] & 0xFFu : 0u;
+    const uint32_t mn2 = j2 < n ? blk[(bl + 2u) * 8u] & 0xFFu : 0u;
+    auto put = [&](uint32_t k, uint32_t word) __attribute__((always_inline)) { o[k] = word; };
+    if (n == B3_ENTRIES) {
+        const uint32_t off = lane * B3_ENTRIES;  // block's first nibble within the span
+        const uint32_t* st = stage[wave];
+        auto nib = [&](uint32_t j) __attribute__((always_inline)) { return (st[(off + j) >> 3] >> (((off + j) & 7u) * 4u)) & 15u; };
+        encode_b3_block(nib, B3_ENTRIES, mn0, mn1, mn2, j1, j2, put);
+    } else {  // the table's last block: read the build table directly
+        auto nib = [&](uint32_t j) __attribute__((always_inline)) { return nib_get(tbl, first + j); };
+        encode_b3_block(nib, n, mn0, mn1, mn2, j1, j2, put);
     }
-    uint32_t best = 0, best_score = 0;
-#pragma unroll
-    for (uint32_t c = 0; c < 4; ++c)
-        if (score[c] > best_score) best_score = score[c], best = c;
-    uint32_t w[8] = {m | (best << 6), 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t q = 0; q < 31; ++q) {
-        uint32_t x = 0;
-        for (int k = 4; k >= 0; --k) {
-            uint32_t j = q * 5u + (uint32_t)k, d = 0;
-            if (j < n) {
-                uint32_t v = b3_source_value(tbl, blk, first + j);
-                d = v <= m + best ? 0u : (v == m + best + 1 ? 1u : 2u);
-            }
-            x = x * 3u + d;
-        }
-        uint32_t byte = 1 + q;
-        w[byte >> 2] |= x << ((byte & 3u) * 8u);
-    }
-    uint4* o = reinterpret_cast<uint4*>(out + i * 8u);
-    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1169,11 +1320,12 @@ hipError_t launch_bfs_seed(int K, const CubeLUT* lut, uint32_t* tbl, const uint3
 hipError_t launch_bfs_layer(int K, bool forward, int grid, const CubeLUT* lut, uint32_t* tbl, const uint32_t* classidx,
                             const uint32_t* class_rep, const uint16_t* class_stab, unsigned long long n_words,
                             unsigned long long n_entries, uint32_t depth, unsigned long long* counter, hipStream_t s) {
+    const int fgrid = max(1, grid * (int)BFS_WG / (int)FWD_WG);  // the same number of waves
 #define CALL(KK)                                                                                                    \
-    if (forward) hipLaunchKernelGGL((bfs_layer_kernel<KK, true>), dim3(grid), dim3(BFS_WG), 0, s, lut, tbl, classidx, \
+    if (forward) hipLaunchKernelGGL((bfs_forward_kernel<KK>), dim3(fgrid), dim3(FWD_WG), 0, s, lut, tbl, classidx,   \
                                     class_rep, class_stab, n_words, n_entries, depth, counter);                     \
-    else hipLaunchKernelGGL((bfs_layer_kernel<KK, false>), dim3(grid), dim3(BFS_WG), 0, s, lut, tbl, classidx,        \
-                            class_rep, class_stab, n_words, n_entries, depth, counter);
+    else hipLaunchKernelGGL((bfs_backward_kernel<KK>), dim3(grid), dim3(BFS_WG), 0, s, lut, tbl, classidx,          \
+                            class_rep, n_words, n_entries, depth, counter);
     CCS_KIND_SWITCH(K, CALL)
 #undef CALL
     return hipGetLastError();
@@ -1196,9 +1348,12 @@ hipError_t launch_gather(int mode, int grid, const uint32_t* blk, unsigned long 
 hipError_t launch_encode(int format, const uint32_t* tbl, const uint32_t* blk, unsigned long long n_entries,
                          unsigned long long b0, unsigned long long nb, uint32_t* out, unsigned int* bad, hipStream_t s) {
     if (nb == 0) return hipSuccess;
+    if (b0 % 64u) return hipErrorInvalidValue;  // a wave's span must start on a 16-byte boundary
+    const unsigned long long n_words = (n_entries + 7u) / 8u;
     dim3 grid((unsigned)((nb + 255) / 256)), block(256);
-    if (format == 0) hipLaunchKernelGGL(encode_block_kernel, grid, block, 0, s, tbl, n_entries, b0, nb, out, bad);
-    else hipLaunchKernelGGL(encode_b3_kernel, grid, block, 0, s, tbl, blk, n_entries, b0, nb, out);
+    if (format == 0)
+        hipLaunchKernelGGL(encode_block_kernel, grid, block, 0, s, tbl, n_words, n_entries, b0, nb, out, bad);
+    else hipLaunchKernelGGL(encode_b3_kernel, grid, block, 0, s, tbl, n_words, blk, n_entries, b0, nb, out);
     return hipGetLastError();
 }
 
