@@ -1,9 +1,10 @@
 """Benchmark: cubes/s on the d4t table, persistent batched form vs plain one-cube-at-a-time.
 
 One step = one batched solve of the workload (4,096 seeded uniform random
-states, seed 3, on d4t, one persistent launch).  Before it: the d4t table is
-built on the GPU (BFS kernels) and the autotuner picks the persistent form's
-knobs on a separate 512-state sample (seed 33); neither is timed.  After it:
+states, seed 3, on d4t, one persistent launch).  Before it: the d4t tables
+(block and base-3) are built from scratch on the GPU, reported per layer as
+`table_build`, and the autotuner picks the persistent form's knobs on a
+separate 512-state sample (seed 33); neither is part of the step.  After it:
 the first 128 states solved one at a time in the plain form, the random-probe
 ceiling of the memory system, a Poisson serving run at 50% and 80% of the
 measured capacity (seed 5), and the CPU oracle on a bounded sample.  Prints
@@ -30,7 +31,7 @@ BYTES_PER_PROBE = 32  # algorithmic: one 32-byte table block per lookup
 # (FETCH_SIZE x2: it books each 128-byte read request as 64 B; WRITE_SIZE x1):
 # profiles/search_kernel_d4t_pmc.md.  A random 32-byte block costs one 128-byte line.
 FETCHED_BYTES_PER_PROBE = 128.3
-WRITTEN_BYTES_PER_PROBE = 0.045
+WRITTEN_BYTES_PER_PROBE = 0.044
 
 
 def log(msg: str) -> None:
