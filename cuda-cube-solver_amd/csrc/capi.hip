@@ -24,8 +24,24 @@ thread_local std::string g_last_error;
 
 namespace ccs {
 
+// The ClassLUT tables must give classidx for every raw corner coordinate;
+// checked once per process, before any table is set up.
+static int class_lut_check() {
+    static int rc = -1;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const CornerClasses& cc = corner_classes();
+        uint32_t bad = 0;
+        for (uint32_t t = 0; t < N_TWIST; ++t)
+            for (uint32_t p = 0; p < N_DPOS; ++p) bad += corner_class(cc.class_lut, t, p) != cc.classidx[t * N_DPOS + p];
+        rc = bad == 0 && cc.class_lut_rows[0] == (int)N_MULTI_TWIST && cc.class_lut_rows[1] == (int)N_MULTI_ORBIT;
+    });
+    return rc;
+}
+
 int tables_setup(ccs_tables* t, const std::string& name) {
     std::string err;
+    if (!class_lut_check()) return fail(CCS_EFAULT, "corner class tables disagree with classidx");
     if (!make_kind(name, t->kind, err)) return fail(CCS_EINVAL, err);
     t->K = kind_K(t->kind);
     if (t->K != 0 && t->K != 8 && t->K != 12 && t->K != 16 && t->K != 24)
@@ -37,6 +53,8 @@ int tables_setup(ccs_tables* t, const std::string& name) {
     HIP_TRY(hipMalloc(&t->d_classidx, cc.classidx.size() * 4));
     HIP_TRY(hipMalloc(&t->d_class_rep, cc.rep.size() * 4));
     HIP_TRY(hipMalloc(&t->d_class_stab, cc.stab.size() * 2));
+    HIP_TRY(hipMalloc(&t->d_class_lut, sizeof(ClassLUT)));
+    HIP_TRY(hipMemcpy(t->d_class_lut, &cc.class_lut, sizeof(ClassLUT), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t->d_lut, &t->lut, sizeof(CubeLUT), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t->d_classidx, cc.classidx.data(), cc.classidx.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t->d_class_rep, cc.rep.data(), cc.rep.size() * 4, hipMemcpyHostToDevice));
@@ -423,18 +441,19 @@ int ccs_index_host(const char* kind, const uint8_t* cubes, int n, uint64_t* idx_
     Kind k;
     std::string err;
     if (!make_kind(kind, k, err)) return fail(CCS_EINVAL, err);
+    if (!class_lut_check()) return fail(CCS_EFAULT, "corner class tables disagree with classidx");
     std::unique_ptr<CubeLUT> L(new CubeLUT());
     build_lut(k, *L);
-    const uint32_t* classidx = corner_classes().classidx.data();
+    const ClassLUT& C = corner_classes().class_lut;  // the search's path to the corner class
     for (int i = 0; i < n; ++i) {
         Packed p = pack(*reinterpret_cast<const Cubie*>(cubes + 40 * (size_t)i));
         Frame f{p.w[0], p.w[1], p.w[2], p.w[3], p.w[4]};
         switch (kind_K(k)) {
-            case 0: idx_out[i] = table_index<0>(*L, classidx, f); break;
-            case 8: idx_out[i] = table_index<8>(*L, classidx, f); break;
-            case 12: idx_out[i] = table_index<12>(*L, classidx, f); break;
-            case 16: idx_out[i] = table_index<16>(*L, classidx, f); break;
-            case 24: idx_out[i] = table_index<24>(*L, classidx, f); break;
+            case 0: idx_out[i] = table_index_c<0>(*L, C, f); break;
+            case 8: idx_out[i] = table_index_c<8>(*L, C, f); break;
+            case 12: idx_out[i] = table_index_c<12>(*L, C, f); break;
+            case 16: idx_out[i] = table_index_c<16>(*L, C, f); break;
+            case 24: idx_out[i] = table_index_c<24>(*L, C, f); break;
             default: return fail(CCS_EINVAL, "unsupported kind");
         }
     }

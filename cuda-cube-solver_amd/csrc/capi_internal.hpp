@@ -61,6 +61,7 @@ struct ccs_tables {
     uint32_t* d_classidx = nullptr;
     uint32_t* d_class_rep = nullptr;
     uint16_t* d_class_stab = nullptr;
+    ccs::ClassLUT* d_class_lut = nullptr;  // corner classes from small tables, for the search's LDS
     uint8_t* d_table[2] = {nullptr, nullptr};
     uint64_t table_bytes[2] = {0, 0};
     unsigned long long* d_prefix[ccs::MAX_PLEN + 1] = {};  // packed prefixes, 8 B per item
@@ -73,6 +74,7 @@ struct ccs_tables {
         hipFree(d_classidx);
         hipFree(d_class_rep);
         hipFree(d_class_stab);
+        hipFree(d_class_lut);
         for (auto* p : d_table) hipFree(p);
         for (auto* p : d_prefix) hipFree(p);
         if (stream) hipStreamDestroy(stream);

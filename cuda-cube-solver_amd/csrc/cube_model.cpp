@@ -243,6 +243,44 @@ void init_statics() {
             if (corner_raw(conjugate(S->ud[s], c)) == cc.rep[k]) st |= (uint16_t)(1u << s);
         cc.stab[k] = st;
     }
+    // The class tables (lut.hpp, ClassLUT): twist and placement transform
+    // separately under each symmetry; the entries are read off classidx.
+    ClassLUT& C = cc.class_lut;
+    memset(&C, 0, sizeof C);
+    std::vector<uint32_t> tw(16 * N_TWIST);
+    for (int s = 0; s < 16; ++s) {
+        for (uint32_t t = 0; t < N_TWIST; ++t) tw[s * N_TWIST + t] = corner_raw(conjugate(S->ud[s], corners_from_raw(t * N_DPOS))) / N_DPOS;
+        for (uint32_t p = 0; p < N_DPOS; ++p)
+            C.ps[s][p] = (uint8_t)(corner_raw(conjugate(S->ud[s], corners_from_raw(p))) % N_DPOS);
+    }
+    std::vector<uint32_t> orbit_row(N_TWIST, 0);
+    uint32_t n_mt = 0, n_mo = 0;
+    for (uint32_t t = 0; t < N_TWIST; ++t) {
+        uint32_t tmin = 0xFFFFFFFFu, s0 = 0, nm = 0;
+        for (int s = 0; s < 16; ++s) tmin = std::min(tmin, tw[s * N_TWIST + t]);
+        for (int s = 15; s >= 0; --s)
+            if (tw[s * N_TWIST + t] == tmin) s0 = (uint32_t)s, ++nm;
+        uint32_t base = 0xFFFFFFFFu;
+        for (uint32_t p = 0; p < N_DPOS; ++p) base = std::min(base, cc.classidx[tmin * N_DPOS + p] >> 4);
+        uint32_t mi = 0, mo = 0;
+        if (nm > 1) {
+            if (n_mt < N_MULTI_TWIST) {
+                mi = ++n_mt;
+                for (uint32_t p = 0; p < N_DPOS; ++p) C.ms[mi - 1][p] = (uint8_t)(cc.classidx[t * N_DPOS + p] & 15u);
+            } else {
+                ++n_mt;  // more than the table holds: the check below fails
+            }
+            if (!orbit_row[tmin]) orbit_row[tmin] = ++n_mo;
+            mo = orbit_row[tmin] <= N_MULTI_ORBIT ? orbit_row[tmin] : 0;
+            if (mo)
+                for (uint32_t p = 0; p < N_DPOS; ++p)
+                    C.rk[mo - 1][C.ps[s0][p]] = (uint8_t)((cc.classidx[t * N_DPOS + p] >> 4) - base);
+        }
+        C.tw[t][0] = s0 | (mi << 4);
+        C.tw[t][1] = base | (mo << 16);
+    }
+    cc.class_lut_rows[0] = (int)n_mt;
+    cc.class_lut_rows[1] = (int)n_mo;
 }
 
 Statics& st() {

@@ -50,6 +50,8 @@ struct CornerClasses {
     std::vector<uint32_t> classidx;  // raw -> class << 4 | sym
     std::vector<uint32_t> rep;       // class -> representative raw
     std::vector<uint16_t> stab;      // class -> stabiliser mask of the representative
+    ClassLUT class_lut;              // the same classes from small tables (lut.hpp)
+    int class_lut_rows[2] = {0, 0};  // several-symmetry twists and orbits found while building it
 };
 const CornerClasses& corner_classes();
 uint32_t corner_raw(const Cubie& c);

@@ -131,12 +131,36 @@ CCS_HD uint32_t comb8_rank(uint32_t m) {
 }
 
 // twist (3^7) * 70 + rank of the D-corner positions (coords.py:112-118).
-CCS_HD uint32_t corner_raw(const CubeLUT& L, const Frame& f) {
+CCS_HD void corner_parts(const Frame& f, uint32_t& twist, uint32_t& pos) {
     uint32_t olo = (f.c0 >> 3) & 0x03030303u, ohi = (f.c1 >> 3) & 0x00030303u;
     // sum of co[i] * 3^i over corners 0..6 (corner 7 is fixed by the others)
-    uint32_t twist = dot4(olo, 0x1B090301u, mul24(dot4(ohi, 0x00090301u, 0u), 81u));
+    twist = dot4(olo, 0x1B090301u, mul24(dot4(ohi, 0x00090301u, 0u), 81u));
     uint32_t dmask = dot4((f.c0 >> 2) & 0x01010101u, 0x08040201u, dot4((f.c1 >> 2) & 0x01010101u, 0x80402010u, 0u));
-    return mul24(twist, 70u) + comb8_rank(dmask);
+    pos = comb8_rank(dmask);
+}
+CCS_HD uint32_t corner_raw(const CubeLUT& L, const Frame& f) {
+    uint32_t twist, pos;
+    corner_parts(f, twist, pos);
+    return mul24(twist, 70u) + pos;
+}
+
+// classidx[twist * 70 + pos] (class << 4 | sym) from the ClassLUT tables
+// (lut.hpp): one read of the twist's entry, then the placement and, for
+// several-symmetry twists, the symmetry, then the rank within a symmetric
+// orbit.  Indices stay in their tables for every lane, also where the value
+// is not used.
+CCS_HD uint32_t corner_class(const ClassLUT& C, uint32_t twist, uint32_t pos) {
+    const uint32_t x = C.tw[twist][0], y = C.tw[twist][1];
+    const uint32_t s0 = x & 15u, mi = x >> 4, mo = y >> 16;
+    const uint32_t q = C.ps[s0][pos];
+    const uint32_t ms = C.ms[mi ? mi - 1u : 0u][pos];
+    const uint32_t rk = C.rk[mo ? mo - 1u : 0u][q];
+    return (((y & 0xFFFFu) + (mo ? rk : q)) << 4) | (mi ? ms : s0);
+}
+CCS_HD uint32_t corner_class(const ClassLUT& C, const Frame& f) {
+    uint32_t twist, pos;
+    corner_parts(f, twist, pos);
+    return corner_class(C, twist, pos);
 }
 
 // K = 0: the 304 kind; K > 0: a group kind with K E-content classes.
@@ -188,6 +212,14 @@ CCS_HD uint32_t edge_coord(const CubeLUT& L, const Frame& f, uint32_t s) {
 template <int K>
 CCS_HD uint64_t table_index(const CubeLUT& L, const uint32_t* __restrict__ classidx, const Frame& f) {
     uint32_t packed = classidx[corner_raw(L, f)];
+    return (uint64_t)(packed >> 4) * KindT<K>::N_EDGE + edge_coord<K>(L, f, packed & 15u);
+}
+
+// The same index with the corner class taken from the ClassLUT tables, as the
+// search computes it.
+template <int K>
+CCS_HD uint64_t table_index_c(const CubeLUT& L, const ClassLUT& C, const Frame& f) {
+    const uint32_t packed = corner_class(C, f);
     return (uint64_t)(packed >> 4) * KindT<K>::N_EDGE + edge_coord<K>(L, f, packed & 15u);
 }
 

@@ -815,13 +815,20 @@ __device__ __forceinline__ Frame load_frame_c(const Frame* p) {
 template <int K, int F, bool QUEUE, bool SERVE>
 __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kernel(SearchArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
+    // LDS: CubeLUT, ClassLUT, then the lanes' stacks (u32 bounds, then u8 moves), lane-major
     CubeLUT& L = *reinterpret_cast<CubeLUT*>(smem);
+    ClassLUT& CL = *reinterpret_cast<ClassLUT*>(smem + sizeof(CubeLUT));
     constexpr uint32_t nt = SEARCH_WG;  // the host launches exactly this many lanes
     const uint32_t tid = threadIdx.x;
-    uint32_t* hst = reinterpret_cast<uint32_t*>(smem + sizeof(CubeLUT)) + tid;
-    uint8_t* pth = reinterpret_cast<uint8_t*>(reinterpret_cast<uint32_t*>(smem + sizeof(CubeLUT)) + STACK_DEPTH * nt) + tid;
-    load_lut(&L, A.lut);
-    const uint32_t* __restrict__ classidx = A.classidx;
+    unsigned char* stk = smem + sizeof(CubeLUT) + sizeof(ClassLUT);
+    uint32_t* hst = reinterpret_cast<uint32_t*>(stk) + tid;
+    uint8_t* pth = reinterpret_cast<uint8_t*>(reinterpret_cast<uint32_t*>(stk) + STACK_DEPTH * nt) + tid;
+    {
+        const uint4* s = reinterpret_cast<const uint4*>(A.class_lut);
+        uint4* d = reinterpret_cast<uint4*>(&CL);
+        for (uint32_t i = tid; i < sizeof(ClassLUT) / 16; i += nt) d[i] = s[i];
+    }
+    load_lut(&L, A.lut);  // ends with a workgroup barrier
     const uint8_t* __restrict__ table = A.table;
 
     Frames6 f;
@@ -1170,7 +1177,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             Frame tf = pick3(f.f0, f.f1, f.f2, kf), ti = pick3(f.f3, f.f4, f.f5, ki);
             right_mul(tf, L.rr[0][kf][m]);
             left_mul(ti, L.lr[0][ki][m]);
-            const uint32_t pkF = classidx[corner_raw(L, tf)], pkI = classidx[corner_raw(L, ti)];
+            const uint32_t pkF = corner_class(CL, tf), pkI = corner_class(CL, ti);
             uint32_t jF, jI;
             const uint8_t* bF = probe_block<K, F>(L, pkF, table, tf, jF);
             const uint8_t* bI = probe_block<K, F>(L, pkI, table, ti, jI);
@@ -1264,7 +1271,9 @@ void* search_fn(int F, int form) {
 
 }  // namespace
 
-size_t search_smem_bytes(int block) { return sizeof(CubeLUT) + (size_t)block * (STACK_DEPTH * 4 + STACK_DEPTH); }
+size_t search_smem_bytes(int block) {
+    return sizeof(CubeLUT) + sizeof(ClassLUT) + (size_t)block * (STACK_DEPTH * 4 + STACK_DEPTH);
+}
 
 hipError_t search_occupancy(int K, int F, int form, int block, int* blocks_per_cu) {
     void* fn = nullptr;

@@ -6,8 +6,10 @@
 
 #include "search_types.hpp"
 
+// Lanes per search workgroup: one workgroup of 12 waves per CU holds one copy
+// of the lookup tables plus every lane's stack in LDS.
 #ifndef SEARCH_WG
-#define SEARCH_WG 256
+#define SEARCH_WG 768
 #endif
 // Waves per SIMD the search kernel is compiled for (caps it at 512 / N VGPRs).
 #ifndef SEARCH_WAVES_PER_SIMD

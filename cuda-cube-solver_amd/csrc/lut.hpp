@@ -64,4 +64,34 @@ struct alignas(16) CubeLUT {
 
 static_assert(sizeof(CubeLUT) % 16 == 0, "LUT copied in 16-byte units");
 
+// Corner class and class symmetry (the classidx word, class << 4 | sym) of a
+// raw corner coordinate twist * 70 + pos, from small tables instead of the
+// 153,090-entry classidx array, so that the search finds them in LDS.
+//
+// Under a UD symmetry s the twist and the D-corner placement transform
+// separately: conj(twist * 70 + pos, s) = tw_s(twist) * 70 + ps[s][pos].  The
+// orbit minimum therefore takes the smallest twist tmin first; M(twist), the
+// symmetries that reach it, is a coset of the stabiliser of tmin, and the
+// class symmetry is the first symmetry of M(twist) that also minimises the
+// placement.  Classes are numbered in order of their representatives' raw
+// coordinate, so a class is base[tmin] plus its placement's rank among tmin's
+// representatives, and that rank depends only on q = ps[s0][pos], s0 being the
+// first symmetry of M(twist).  The host checks these tables against classidx
+// for every raw coordinate.
+constexpr uint32_t N_TWIST = 2187;
+constexpr uint32_t N_DPOS = 70;
+constexpr uint32_t N_MULTI_TWIST = 363;  // twists reaching their minimum through several symmetries
+constexpr uint32_t N_MULTI_ORBIT = 54;   // minimal twists with a nontrivial stabiliser
+
+struct alignas(16) ClassLUT {
+    // per twist: x = s0 | mi << 4 (mi: 1 + row of ms, 0 if s0 is the only symmetry);
+    // y = base[tmin] | mo << 16 (mo: 1 + row of rk, 0 if tmin's stabiliser is trivial)
+    uint32_t tw[N_TWIST][2];
+    uint8_t ms[N_MULTI_TWIST][N_DPOS];  // class symmetry of (twist, pos) for several-symmetry twists
+    uint8_t ps[16][N_DPOS];             // D-corner placement rank after symmetry s
+    uint8_t rk[N_MULTI_ORBIT][N_DPOS];  // rank of the representative among tmin's, by q
+    uint8_t pad[2];
+};
+static_assert(sizeof(ClassLUT) % 16 == 0, "LUT copied in 16-byte units");
+
 }  // namespace ccs

@@ -75,7 +75,8 @@ struct ServeDone {
 
 struct SearchArgs {
     const CubeLUT* lut;
-    const uint32_t* classidx;
+    const ClassLUT* class_lut;  // corner classes for the probes (copied to LDS)
+    const uint32_t* classidx;   // the same classes as one array (admission's root bounds)
     const uint8_t* table;
     SearchJob* jobs;
     // plain form: one round of the listed cubes

@@ -200,6 +200,7 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     SearchArgs A;
     memset(&A, 0, sizeof A);
     A.lut = t->d_lut;
+    A.class_lut = t->d_class_lut;
     A.classidx = t->d_classidx;
     A.table = t->d_table[t->format];
     A.jobs = s->d_jobs;

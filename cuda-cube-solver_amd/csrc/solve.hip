@@ -154,6 +154,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     SearchArgs A;
     memset(&A, 0, sizeof A);
     A.lut = t->d_lut;
+    A.class_lut = t->d_class_lut;
     A.classidx = t->d_classidx;
     A.table = t->d_table[t->format];
     A.jobs = d_jobs;

@@ -21,7 +21,7 @@ from . import states
 from .solver import GpuSolver, Tables
 
 CHUNK_NODES = (65536, 262144, 1048576)
-BLOCKS_PER_CU = (0, 2)       # 0: as many as the occupancy calculator allows
+BLOCKS_PER_CU = (0,)         # 0: as many as the occupancy calculator allows (one 768-lane workgroup per CU)
 PREFIX_LEN = (4, 5)
 DEEPER_AT = ((5e7, 3e9), (1e7, 1e9))
 

@@ -47,6 +47,15 @@ def test_kernel_lut_arithmetic_matches_reference_index(index_vectors, name):
     assert (out == np.array(index_vectors["kinds"][name]["table_index"], dtype=np.uint64)).all()
 
 
+def test_corner_class_tables_agree_with_classidx():
+    """The search reads corner classes from small LDS tables (twist entry,
+    placement under a symmetry, rank within a symmetric orbit).  The library
+    checks them against the classidx array for all 153,090 raw corner
+    coordinates before its first use and refuses to run otherwise; the
+    reference's classidx is pinned by its SHA-256 in test_oracle."""
+    assert _lib.lib().ccs_index_host(b"d4t", None, 0, None) == 0, _lib.lib().ccs_last_error()
+
+
 def test_kernel_lut_arithmetic_matches_oracle_on_seeded_states():
     sts = states.uniform_states(300, 99) + states.flip_heavy_states(20, 4)
     arr = O.cubes([(s.cp, s.co, s.ep, s.eo) for s in sts])
