@@ -124,7 +124,7 @@ typedef struct {
     int32_t mode;           /* 0: plain (per-round launches, the reference's form); 1: persistent batched */
     int32_t prefix_len;     /* work-item prefix length, >= 4 (solver.py:119, default 4) */
     double deeper_at[2];    /* longer prefixes after rounds of this many nodes (solver.py:123: 5e7, 3e9) */
-    int32_t block;          /* workgroup size (0: 256) */
+    int32_t block;          /* workgroup size (0: 768, the only size the kernels are built for) */
     int32_t blocks_per_cu;  /* persistent grid per CU (0: from occupancy) */
     int32_t max_bound;      /* give up above this bound (0: 24); searches stop at 20, the longest optimal solution of any legal cube */
     int32_t poll_interval;  /* picks between polls of the found flag, power of two (0: 64) */
