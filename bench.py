@@ -244,8 +244,17 @@ def main() -> None:
                      "traffic": round(probes / args.steps * (FETCHED_BYTES_PER_PROBE + WRITTEN_BYTES_PER_PROBE))},
     }
     line["table_build"] = build_summary(build_log, counts, build_s)
+    # The measurements after the step are reported beside the metric; one that
+    # fails is reported as an error and does not cost the line.
+    def extra(name, fn):
+        try:
+            return fn()
+        except Exception as e:  # noqa: BLE001
+            log(f"{name} failed: {type(e).__name__}: {e}")
+            return {"error": f"{type(e).__name__}: {e}"}
+
     if not args.no_gather:
-        line["probe_ceiling"] = gather_ceiling(tb)
+        line["probe_ceiling"] = extra("probe ceiling", lambda: gather_ceiling(tb))
     if not args.no_serve:
         from cuda_cube_solver_amd import serve_bench
 
@@ -253,12 +262,19 @@ def main() -> None:
         line["serving"] = []
         for frac in (0.5, 0.8):
             qps = frac * end_to_end
-            r = serve_bench.run(tb, qps, max(16, int(qps * args.serve_seconds)), seed=5, hard_pool=hard, log=log)
-            r.pop("lengths")
+
+            def one():
+                r = serve_bench.run(tb, qps, max(16, int(qps * args.serve_seconds)), seed=5, hard_pool=hard,
+                                    log=log)
+                r.pop("lengths")
+                return r
+
+            r = extra(f"serving at {qps:.0f} QPS", one)
             r["load"] = f"{frac:.0%} of the batched capacity"
             line["serving"].append(r)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(tb, cubes, nodes_total / (args.cubes * args.steps), 20.0)
+        line["cpu_baseline"] = extra("CPU baseline", lambda: cpu_baseline(
+            tb, cubes, nodes_total / (args.cubes * args.steps), 20.0))
     print(json.dumps(line), flush=True)
 
 

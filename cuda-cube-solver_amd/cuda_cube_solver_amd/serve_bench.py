@@ -20,7 +20,9 @@ from .solver import Tables
 def request_mix(n: int, seed: int, hard_pool: list) -> list:
     rng = states.SplitMix64(seed)
     uni = states.uniform_states(n, seed + 1000)
-    flips = states.flip_heavy_states(max(n // 20 + 2, 2), seed + 2000)[:-1]  # the superflip is not in the mix
+    # the superflip (all 12 edges flipped) is not in the mix: the seeded list ends
+    # with it and a random even subset can also be all 12
+    flips = [f for f in states.flip_heavy_states(max(n // 20 + 2, 2), seed + 2000) if f.eo != (1,) * 12]
     out = []
     for i in range(n):
         r = rng.below(100)
@@ -58,8 +60,10 @@ def run(tables: Tables, qps: float, n: int, seed: int = 5, hard_pool: list | Non
     with Server(tables, state_budget_bytes=state_budget_bytes, **server_kw) as srv:
         done = threading.Event()
 
+        quit = threading.Event()
+
         def collector():
-            while len(results) < n:
+            while len(results) < n and not quit.is_set():
                 for r in srv.poll(256, 200):
                     results[r.tag] = r
             done.set()
@@ -73,7 +77,10 @@ def run(tables: Tables, qps: float, n: int, seed: int = 5, hard_pool: list | Non
                 time.sleep(delay)
             srv.submit(cube, tag=i)
         window = time.perf_counter() - t0
-        if not done.wait(timeout=max(120.0, 4 * n / qps)):
+        ok = done.wait(timeout=max(120.0, 4 * n / qps))
+        quit.set()
+        th.join(timeout=5)  # the collector must not poll a stopped server
+        if not ok:
             raise TimeoutError(f"{len(results)}/{n} served")
         wall = time.perf_counter() - t0
     lat = [results[i].latency_ms for i in range(n)]
