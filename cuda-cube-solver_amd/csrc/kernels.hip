@@ -610,15 +610,16 @@ __device__ __forceinline__ unsigned long long make_ticket(unsigned long long kee
 }
 // Items per fetch for a round: about chunk_nodes nodes per chunk, estimated from
 // the previous round's nodes and the usual growth of ~13x per bound, but at
-// least enough chunks for this cube's share of the launch's lanes (and never
-// fewer than 4096), so that the last cubes of a batch still fill the GPU.
+// least CHUNKS_PER_LANE chunks per lane of this cube's share of the launch's
+// lanes (and never fewer than 4096), so that the last cubes of a batch still
+// fill the GPU and no lane is left holding many of a round's last items.
 __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned long long last_nodes,
                                                   uint32_t n_items) {
     unsigned long long per_item = (last_nodes * 13u) / (n_items ? n_items : 1u) + 1u;
     // cubes admitted and not yet finished (u32 differences stay right across wraparound)
     unsigned long long live = A.serving ? (unsigned long long)((uint32_t)ld_c(&A.q->mb_head) - ld_c(&A.q->n_final))
                                         : (unsigned long long)(A.q->n_jobs - ld_c(&A.q->n_final));
-    unsigned long long want = A.n_lanes / (live ? live : 1ull);
+    unsigned long long want = (CHUNKS_PER_LANE * A.n_lanes) / (live ? live : 1ull);
     want = want < 4096ull ? 4096ull : want;
     uint32_t lg = 0;
     while (lg < 6 && (per_item << (lg + 1)) <= A.chunk_nodes && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
@@ -657,6 +658,7 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
         cur = prev;
     }
     ring_push(A, nb, c);
+    st_c(&A.q->split_want, 0u);  // new work in the ring: no more donations needed
 }
 
 __device__ __forceinline__ bool round_drained(const SearchArgs& A, unsigned long long t) {
@@ -750,11 +752,23 @@ __device__ __forceinline__ void report_solution(const SearchArgs& A, SearchJob* 
 // Serving: take one mailbox record into its slot: root bounds of the six
 // frames, a fresh round ticket (keeping the in-flight count of late
 // fetchers), and a ring entry.  Returns false when the mailbox is empty.
+//
+// The tail lives in host memory, so reading it crosses the bus: idle waves
+// poll here, and the whole grid reads it at most once per mb_poll_ticks,
+// keeping the last value read in mb_seen.  The read is relaxed: the record
+// itself is read with system-scope loads that bypass the caches, and an
+// acquire at system scope would invalidate the XCD's L2 on every poll.
 template <int K, int F>
 __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
     unsigned long long h = ld_c(&A.q->mb_head);
-    unsigned long long tail = __hip_atomic_load(A.mb_tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-    if (h >= tail) return false;
+    unsigned long long tail = ld_c(&A.q->mb_seen);
+    if (h >= tail) {
+        const unsigned long long now = wall_clock64(), last = ld_c(&A.q->mb_polled);
+        if (now - last < A.mb_poll_ticks || atomicCAS(&A.q->mb_polled, last, now) != last) return false;
+        tail = sys_load(A.mb_tail);
+        atomicMax(&A.q->mb_seen, tail);
+        if (h >= tail) return false;
+    }
     if (atomicCAS(&A.q->mb_head, h, h + 1) != h) return true;  // another wave took it
     const ServeRec* r = &A.mb[h % A.mb_size];
     uint32_t c = sys_load(&r->slot);
@@ -790,6 +804,7 @@ __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
         cur = prev;
     }
     ring_push(A, bound, c);
+    st_c(&A.q->split_want, 0u);
     return true;
 }
 
@@ -802,6 +817,129 @@ __device__ __forceinline__ void end_chunk(const SearchArgs& A, uint32_t c, Searc
     chunk_nodes = 0;
     complete();
     release_items(A, c, 1);
+}
+
+// Donated subtrees.  Once no cube has items left to hand out, lanes that find
+// no work raise split_want; every SPLIT_CHECK passes each wave looks at it, and
+// a lane that has spent SPLIT_MIN_NODES nodes on its item (so the item is a
+// long one) hands the untried moves of its shallowest level, if they are at
+// least SPLIT_MIN_LEFT moves from the bound, to a SplitRec: that level is then
+// done for it, so its item root moves below it.  Donating short items as well
+// costs more in replays (and their probes) than it gains.  The taker replays the path
+// (its moves are not counted again: node counts stay those of an undivided
+// round) and searches the node from the move after the donor's one.  A record
+// holds one in-flight share of the round until its taker releases it, so a
+// round still closes only when all of its work has drained.
+constexpr uint32_t SPLIT_CHECK = 512;  // passes between two looks of a wave (a power of two)
+constexpr int SPLIT_MIN_LEFT = 7;
+constexpr unsigned long long SPLIT_MIN_NODES = 2048;
+constexpr int SPLIT_MAX_K = 7;         // deepest donated node: its path and skip byte fill one u64
+
+//
+// Returns the lane's new item root, or -1 if it donated nothing; a root below
+// the current depth g means that the node's untried moves went too (the
+// caller clears them).  Out of line, like take_split.
+__device__ __attribute__((noinline)) int donate(QueueCtl* q, SplitRec* splits, const CubeLUT& L, SearchJob* J, uint32_t c,
+                                                uint32_t bound, int g, int m, uint32_t todo, int root, int phase,
+                                                uint32_t found_seen, unsigned long long nodes, const uint32_t* hst,
+                                                const uint8_t* pth, uint32_t nt) {
+    int k = -1;
+    uint32_t p = 0;
+    if (phase == 3 && !found_seen && nodes >= SPLIT_MIN_NODES) {
+        int d = root;
+        for (; d < g && d <= SPLIT_MAX_K && (int)bound - d >= SPLIT_MIN_LEFT; ++d) {
+            const uint32_t pd = pth[d * nt];
+            const int ld = d > 0 ? (int)pth[(d - 1) * nt] : -1;
+            const uint32_t rem = L.amask[ld + 1] & ~axis_mask(excl_axes(hst[d * nt], bound - (uint32_t)d)) &
+                                 ~((2u << pd) - 1u);
+            if (rem) {
+                k = d;
+                p = pd;
+                break;
+            }
+        }
+        // every level above is done: the current node's untried moves (the one
+        // under evaluation, m, stays with the donor)
+        if (k < 0 && d == g && m >= 0 && todo && g <= SPLIT_MAX_K && (int)bound - g >= SPLIT_MIN_LEFT) {
+            k = g;
+            p = (uint32_t)m;
+        }
+    }
+    const unsigned long long cm = __ballot(k >= 0);
+    if (!cm) return -1;
+    const uint32_t n = (uint32_t)__popcll(cm);
+    const int leader = (int)__builtin_ctzll(cm);
+    uint32_t granted = 0, blo = 0, bhi = 0;
+    if ((int)lane_id() == leader) {
+        const uint32_t old = atomicAdd(&q->split_pend, n);
+        granted = old < SPLIT_PEND_MAX ? min(n, SPLIT_PEND_MAX - old) : 0u;
+        if (granted < n) atomicSub(&q->split_pend, n - granted);
+        if (granted) {
+            const unsigned long long b = atomicAdd(&q->split_tail, (unsigned long long)granted);
+            blo = (uint32_t)b;
+            bhi = (uint32_t)(b >> 32);
+        }
+    }
+    granted = __builtin_amdgcn_readlane(granted, leader);
+    blo = __builtin_amdgcn_readlane(blo, leader);
+    bhi = __builtin_amdgcn_readlane(bhi, leader);
+    const uint32_t rank = (uint32_t)__popcll(cm & lanemask_lt());
+    if (k < 0 || rank >= granted) return -1;
+    const unsigned long long slot = (((unsigned long long)bhi << 32) | blo) + rank;
+    SplitRec* R = &splits[slot & (SPLIT_RING - 1u)];
+    unsigned long long path = (unsigned long long)(p + 1u) << (8 * k);
+    for (int j = 0; j < k; ++j) path |= (unsigned long long)pth[j * nt] << (8 * j);
+    atomicAdd(&J->ticket, T_INF_ONE);  // the record's share of the round, released by its taker
+    st_c(&R->path, path);
+    st_c(&R->cube, c);
+    st_c(&R->info, bound | ((uint32_t)k << 8));
+    complete();
+    st_c(&R->seq, slot + 1ull);
+    // k < g: the levels down to k are done; k == g: the node keeps only m, and
+    // the item ends once m is done (or pruned)
+    return k < g ? k + 1 : g + 1;
+}
+
+// A fetching lane's donated subtree, if one is published: each lane reads the
+// record it would take before the wave claims the run of published ones with
+// one CAS on the head.  While the head stays put no writer can reach those
+// records (at most SPLIT_PEND_MAX are reserved past it).  Called by the lanes
+// that fetch, together; out of line, so that this rarely used code does not
+// add to the search loop's registers.
+struct SplitTake {
+    unsigned long long path;
+    uint32_t cube, info;
+    bool ok;
+};
+__device__ __attribute__((noinline)) SplitTake take_split(QueueCtl* q, const SplitRec* splits) {
+    SplitTake r{0ull, 0u, 0u, false};
+    const unsigned long long smask = __ballot(1);
+    const uint32_t srank = (uint32_t)__popcll(smask & lanemask_lt());
+    unsigned long long sh = 0, stl = 0;
+    if (srank == 0) {
+        sh = ld_c(&q->split_head);
+        stl = ld_c(&q->split_tail);
+    }
+    sh = first_lane_u64(sh);
+    stl = first_lane_u64(stl);
+    if (sh >= stl) return r;
+    const unsigned long long sl = sh + srank;
+    const SplitRec* R = &splits[sl & (SPLIT_RING - 1u)];
+    const bool ok = sl < stl && ld_c(&R->seq) == sl + 1ull;
+    if (ok) {
+        r.path = ld_c(&R->path);
+        r.cube = ld_c(&R->cube);
+        r.info = ld_c(&R->info);
+    }
+    const unsigned long long bad = __ballot(!ok);
+    const uint32_t n = bad ? (uint32_t)__popcll(smask & ((1ull << __builtin_ctzll(bad)) - 1ull))
+                           : (uint32_t)__popcll(smask);
+    uint32_t won = 0;
+    if (srank == 0 && n) won = atomicCAS(&q->split_head, sh, sh + n) == sh;
+    won = first_lane_u32(won);
+    if (won && srank == 0) atomicSub(&q->split_pend, n);
+    r.ok = won && srank < n;
+    return r;
 }
 
 __device__ __forceinline__ Frame load_frame_c(const Frame* p) {
@@ -837,6 +975,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     SearchJob* J = nullptr;
     uint32_t bound = 0, todo = 0, ord = 0, idle = 0, plen = 0;
     int g = 0, tgt = 0, m = -1;
+    // Depth of the item's root: the search below it ends when the lane is back
+    // at (or above) it.  The prefix length for a plain item; deeper once the
+    // lane has donated the rest of the levels above.
+    int root = 0;
+    uint32_t wtick = 0;  // persistent form: passes of this wave
     // Child m under evaluation: forward / inverse frames still to probe (bits
     // in frame_order's layout), whether a probe pruned it, whether it is done.
     uint32_t remF = 0, remI = 0;
@@ -862,6 +1005,19 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     bool done = false;
 
     for (;;) {
+        if (QUEUE) {
+            // Donation: every SPLIT_CHECK passes the wave looks at whether lanes are
+            // out of work (one load, a wait of about a microsecond in some
+            // milliseconds of passes); if so, each lane deep in a subtree hands the
+            // untried moves of its shallowest level to them (a SplitRec) and keeps
+            // the rest.
+            wtick = __builtin_amdgcn_readfirstlane(wtick + 1u);
+            if ((wtick & (SPLIT_CHECK - 1u)) == 0u && __builtin_amdgcn_readfirstlane(ld_c(&A.q->split_want))) {
+                const int nr = donate(A.q, A.splits, L, J, c, bound, g, m, todo, root, phase, found_seen, nodes, hst, pth, nt);
+                if (nr > g) todo = 0;
+                if (nr >= 0) root = nr;
+            }
+        }
         bool eval = false;
         // The pass's frame update, applied after the pass for every lane at once
         // (push: move mv; pop: undo move mv): -1 none.  `leaf`: the push reached
@@ -896,7 +1052,12 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         todo = L.amask[m + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
                         ord = frame_order(hp, bound - (uint32_t)g);
                     }
-                    if ((uint32_t)g == plen) phase = 3;  // the prefix is in place
+                    if (phase == 2 && g == root) {
+                        // the prefix is in place; a donated subtree resumes after the
+                        // donor's move, whose successor is in the byte past its path
+                        phase = 3;
+                        todo &= ~((1u << (uint32_t)((pbits >> (8 * g)) & 0xFFu)) - 1u);
+                    }
                 } else if (phase == 2) {  // a prefix move was pruned: the item ends
                     hfail = g;
                     phase = 4;
@@ -906,12 +1067,12 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             }
             bool pop = phase == 1;
             if (phase == 3 && todo == 0) {  // node exhausted: the item is complete, or back up
-                if ((uint32_t)g == plen || found_seen) phase = 4;
+                if (g <= root || found_seen) phase = 4;
                 else pop = true;
             }
             if (phase == 4) {
                 // ---- the item is finished: account its nodes ----
-                hold = (uint32_t)g <= plen ? g : -1;  // deeper only when found ended the search
+                hold = g <= root ? g : -1;  // deeper only when found ended the search
                 if (probes >= 0x80000000u && A.stats) {
                     atomicAdd(&A.stats[0], (unsigned long long)probes);
                     probes = 0;
@@ -931,6 +1092,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 // ---- fetch a work item: (cube, bound, prefix length, prefix) ----
                 uint32_t item;
                 bool fresh;
+                bool spl = false;  // a donated subtree (its path in spath)
+                unsigned long long spath = 0;
                 if (!QUEUE) {
                     unsigned long long mask = __ballot(1);
                     uint32_t rank = __popcll(mask & lanemask_lt());
@@ -965,95 +1128,118 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     item_cur = item;
                 } else {
                     if (idle && (--idle)) break;  // back off between empty fetches
-                    unsigned long long mask = __ballot(1);
-                    uint32_t cnt = __popcll(mask), rank = __popcll(mask & lanemask_lt());
-                    uint32_t cc = 0xFFFFFFFFu, clog = 0;
-                    unsigned long long old = 0;
-                    // serving: a stop request ends the launch within a few fetches even
-                    // while cubes remain.  The fetch count is kept wave-uniform (each
-                    // fetch takes its leader's count + 1) and the flag, in host memory,
-                    // is read on every 4th fetch.
-                    if (SERVE) atick = first_lane_u32(atick) + 1u;
-                    const bool stopping = SERVE && rank == 0 && (atick & 3u) == 0u && sys_load(A.host_stop);
-                    if (rank == 0 && !stopping) {
-                        for (int tries = 0; tries < 32; ++tries) {
-                            unsigned long long h = ld_c(&A.q->head), t = ld_c(&A.q->tail);
-                            // serving: take new cubes when the ring is empty, and while
-                            // fewer than ADMIT_BELOW rounds are queued on every 16th fetch
-                            // (the mailbox is in host memory; its reads cross the bus)
-                            if (SERVE && (h >= t || (t - h < ADMIT_BELOW && (atick & 15u) == 1u && tries == 0)) &&
-                                admit_one<K, F>(A, L))
-                                continue;
-                            if (h >= t) break;
-                            unsigned long long e = ld_c(&A.ring[h % A.ring_size]);
-                            if ((e >> 37) != (h & RING_SEQ_MASK)) break;  // entry still being published
-                            uint32_t ce = (uint32_t)e, eb = (uint32_t)(e >> 32) & 31u;
-                            SearchJob* Je = &A.jobs[ce];
-                            unsigned long long tk = ld_c(&Je->ticket);
-                            if ((tk & (T_CLOSING | T_RETIRED)) || t_bound(tk) != eb ||
-                                t_next(tk) >= A.pre_n[t_plen(tk)] || ld_c(&Je->found)) {
-                                atomicCAS(&A.q->head, h, h + 1);  // stale, finished or fully handed out
-                                continue;
+                    {
+                        unsigned long long mask = __ballot(1);
+                        uint32_t cnt = __popcll(mask), rank = __popcll(mask & lanemask_lt());
+                        uint32_t cc = 0xFFFFFFFFu, clog = 0;
+                        unsigned long long old = 0;
+                        // serving: a stop request ends the launch within a few fetches even
+                        // while cubes remain.  The fetch count is kept wave-uniform (each
+                        // fetch takes its leader's count + 1) and the flag, in host memory,
+                        // is read on every 4th fetch.
+                        if (SERVE) atick = first_lane_u32(atick) + 1u;
+                        const bool stopping = SERVE && rank == 0 && (atick & 3u) == 0u && sys_load(A.host_stop);
+                        if (rank == 0 && !stopping) {
+                            for (int tries = 0; tries < 32; ++tries) {
+                                unsigned long long h = ld_c(&A.q->head), t = ld_c(&A.q->tail);
+                                // serving: take new cubes when the ring is empty, and while
+                                // fewer than ADMIT_BELOW rounds are queued on every 16th fetch
+                                // (the mailbox is in host memory; its reads cross the bus)
+                                if (SERVE && (h >= t || (t - h < ADMIT_BELOW && (atick & 15u) == 1u && tries == 0)) &&
+                                    admit_one<K, F>(A, L))
+                                    continue;
+                                if (h >= t) break;
+                                unsigned long long e = ld_c(&A.ring[h % A.ring_size]);
+                                if ((e >> 37) != (h & RING_SEQ_MASK)) break;  // entry still being published
+                                uint32_t ce = (uint32_t)e, eb = (uint32_t)(e >> 32) & 31u;
+                                SearchJob* Je = &A.jobs[ce];
+                                unsigned long long tk = ld_c(&Je->ticket);
+                                if ((tk & (T_CLOSING | T_RETIRED)) || t_bound(tk) != eb ||
+                                    t_next(tk) >= A.pre_n[t_plen(tk)] || ld_c(&Je->found)) {
+                                    atomicCAS(&A.q->head, h, h + 1);  // stale, finished or fully handed out
+                                    continue;
+                                }
+                                clog = t_chunk(tk);
+                                old = atomicAdd(&Je->ticket, (unsigned long long)cnt * (T_INF_ONE + (1ull << clog)));
+                                // Round robin: the cube moves behind the others, so each live cube
+                                // gets about its share of the lanes and its round is searched
+                                // roughly in item order (a final round then stops near where a
+                                // sequential search would).  Only the wave that advances the head
+                                // re-queues the entry, and only while items remain.
+                                if (atomicCAS(&A.q->head, h, h + 1) == h &&
+                                    t_next(old) + ((unsigned long long)cnt << clog) < A.pre_n[t_plen(old)])
+                                    ring_push(A, eb, ce);
+                                cc = ce;
+                                break;
                             }
-                            clog = t_chunk(tk);
-                            old = atomicAdd(&Je->ticket, (unsigned long long)cnt * (T_INF_ONE + (1ull << clog)));
-                            // Round robin: the cube moves behind the others, so each live cube
-                            // gets about its share of the lanes and its round is searched
-                            // roughly in item order (a final round then stops near where a
-                            // sequential search would).  Only the wave that advances the head
-                            // re-queues the entry, and only while items remain.
-                            if (atomicCAS(&A.q->head, h, h + 1) == h &&
-                                t_next(old) + ((unsigned long long)cnt << clog) < A.pre_n[t_plen(old)])
-                                ring_push(A, eb, ce);
-                            cc = ce;
+                        }
+                        cc = first_lane_u32(cc);
+                        old = first_lane_u64(old);
+                        clog = first_lane_u32(clog);
+                        // no round has items left: a donated subtree, if one is published
+                        const SplitTake st = cc == 0xFFFFFFFFu ? take_split(A.q, A.splits) : SplitTake{0ull, 0u, 0u, false};
+                        if (cc == 0xFFFFFFFFu && !st.ok) {
+                            if (ld_c(&A.q->n_final) >= A.q->n_jobs || ld_c(&A.q->stop) ||
+                                (SERVE && sys_load(A.host_stop))) {
+                                done = true;
+                            } else {
+                                // out of work: ask for donations, and back off, longer once
+                                // the wait has lasted (idle waves then poll the queue words
+                                // far less often)
+                                if (rank == 0 && !ld_c(&A.q->split_want)) st_c(&A.q->split_want, 1u);
+                                unsigned long long now = wall_clock64();
+                                idle = idle_since && now - idle_since > A.backoff_ticks ? 1024 : 64;
+                                // watchdog: a lane that finds no work for idle_limit ends the launch
+                                if (!idle_since) idle_since = now;
+                                else if (now - idle_since > A.idle_limit) atomicMax(&A.q->stop, 2u);
+                            }
                             break;
                         }
-                    }
-                    cc = first_lane_u32(cc);
-                    old = first_lane_u64(old);
-                    clog = first_lane_u32(clog);
-                    if (cc == 0xFFFFFFFFu) {
-                        if (ld_c(&A.q->n_final) >= A.q->n_jobs || ld_c(&A.q->stop) ||
-                            (SERVE && sys_load(A.host_stop))) {
-                            done = true;
+                        idle_since = 0;
+                        if (st.ok) {
+                            spl = true;
+                            spath = st.path;
+                            c = st.cube;
+                            J = &A.jobs[c];
+                            bound = st.info & 31u;
+                            plen = (st.info >> 8) & 31u;  // the path: replayed like a prefix
+                            item = 0;
+                            item_cur = 0;
+                            fresh = true;
+                            hold = -1;
+                            chunk_left = 0;
+                            chunk_nodes = 0;
                         } else {
-                            idle = 64;
-                            // watchdog: a lane that finds no work for idle_limit ends the launch
-                            unsigned long long now = wall_clock64();
-                            if (!idle_since) idle_since = now;
-                            else if (now - idle_since > A.idle_limit) atomicMax(&A.q->stop, 2u);
+                            const uint32_t pl = t_plen(old);
+                            const uint32_t n_items = A.pre_n[pl];
+                            item = t_next(old) + (rank << clog);
+                            bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < n_items;
+                            unsigned long long inval = __ballot(!valid);
+                            if (rank == 0 && inval) release_items(A, cc, __popcll(inval));
+                            if (!valid) continue;
+                            c = cc;
+                            J = &A.jobs[c];
+                            bound = t_bound(old);
+                            plen = pl;
+                            item = (item >> clog) * min(ITEM_BLOCK, 1u << clog);  // chunk's first item
+                            item_cur = item;
+                            fresh = true;  // a chunk may follow a reuse of the slot: start from the root
+                            hold = -1;
+                            chunk_left = ((1u << clog) - 1u) | (clog << 8);
+                            chunk_nodes = 0;
+                            if (ld_c(&J->found)) {
+                                chunk_left = 0;
+                                end_chunk(A, c, J, chunk_nodes);
+                                continue;
+                            }
                         }
-                        break;
-                    }
-                    idle_since = 0;
-                    const uint32_t pl = t_plen(old);
-                    const uint32_t n_items = A.pre_n[pl];
-                    item = t_next(old) + (rank << clog);
-                    bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < n_items;
-                    unsigned long long inval = __ballot(!valid);
-                    if (rank == 0 && inval) release_items(A, cc, __popcll(inval));
-                    if (!valid) continue;
-                    c = cc;
-                    J = &A.jobs[c];
-                    bound = t_bound(old);
-                    plen = pl;
-                    item = (item >> clog) * min(ITEM_BLOCK, 1u << clog);  // chunk's first item
-                    item_cur = item;
-                    fresh = true;  // a chunk may follow a reuse of the slot: start from the root
-                    hold = -1;
-                    chunk_left = ((1u << clog) - 1u) | (clog << 8);
-                    chunk_nodes = 0;
-                    if (ld_c(&J->found)) {
-                        chunk_left = 0;
-                        end_chunk(A, c, J, chunk_nodes);
-                        continue;
                     }
                 }
                 // ---- start the item: its prefix is replayed move by move through
                 // the probe step below, all six frames (kernels.cu:1552-1614) ----
                 unsigned long long nb;
                 if (QUEUE) {
-                    nb = fresh ? A.pre_tab[plen][item] : pnext;
+                    nb = spl ? spath : fresh ? A.pre_tab[plen][item] : pnext;
                     if (chunk_left & 0xFFu) {  // the next item's prefix, used when it starts
                         const uint32_t n_items = A.pre_n[plen];
                         const uint32_t nx = next_item(item, chunk_left & 0xFFu, chunk_left >> 8, n_items);
@@ -1090,11 +1276,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     ord = frame_order(hp, bound);
                 }
                 // moves shared with the held state count as replayed (node counts are
-                // those of a replay from the root)
-                nodes = (unsigned long long)shared;
+                // those of a replay from the root); a donated subtree's path was
+                // counted by its donor
+                nodes = spl ? 0ull - plen : (unsigned long long)shared;
                 tgt = shared;  // undo down to the shared depth first
                 todo = 0;
-                phase = g > tgt ? 1 : ((uint32_t)g < plen ? 2 : 3);
+                root = (int)plen;
+                phase = g > tgt ? 1 : (g < root ? 2 : 3);
                 if (phase == 3) todo = L.amask[0] & ~axis_mask(excl_axes(hp, bound));  // empty prefix
                 pop = phase == 1;
             }

@@ -17,6 +17,10 @@ constexpr unsigned long long DEFAULT_CHUNK_NODES = 262144;
 // from waiting seconds on a single lane.
 constexpr unsigned long long SERVE_CHUNK_NODES = 16384;
 constexpr int MAX_PLEN = 7;     // longest work-item prefix
+// Persistent form: a round is cut into at least this many chunks per lane of
+// its cube's share of the launch (a lone cube then takes 1-2 items per fetch
+// at 7.7M items).
+constexpr unsigned long long CHUNKS_PER_LANE = 16;
 
 // One cube being searched, resident in HBM (the role of the reference's
 // CubeJob, kernels.cu:702-710, plus the per-cube round state of the
@@ -49,7 +53,31 @@ struct QueueCtl {
     unsigned int pad;
     unsigned long long mb_head;  // serving: mailbox records admitted
     unsigned long long done_next;// serving: completion records claimed
+    unsigned long long mb_seen;  // serving: the mailbox tail as last read from host memory
+    unsigned long long mb_polled;// serving: device clock of that read
+    // donated subtrees (SplitRec ring): records claimed, records reserved,
+    // records reserved and not yet claimed, and whether lanes are out of work
+    unsigned long long split_head;
+    unsigned long long split_tail;
+    unsigned int split_pend;
+    unsigned int split_want;
 };
+
+// A subtree donated by a lane to lanes without work once no cube has items
+// left to hand out: the node reached by `path` (move j in byte j, j < k),
+// searched from the move after p on; byte k of `path` holds p + 1.
+struct SplitRec {
+    unsigned long long path;
+    uint32_t cube;
+    uint32_t info;             // round bound | k << 8
+    unsigned long long seq;    // ring index + 1 once the record is written
+    unsigned long long pad;
+};
+// Ring of donated subtrees (a power of two) and the most reserved and not yet
+// claimed at once; half the ring, so a writer never reaches a record that a
+// reader may still be reading.
+constexpr unsigned int SPLIT_RING = 16384;
+constexpr unsigned int SPLIT_PEND_MAX = SPLIT_RING / 2;
 
 // Serving mailbox record (host -> device, in pinned host memory); seq is
 // written last, as record index + 1.
@@ -97,6 +125,7 @@ struct SearchArgs {
     unsigned long long deeper0, deeper1;
     unsigned int poll_mask;      // found flag polled every poll_mask + 1 picks
     unsigned long long idle_limit;  // persistent form: device-clock ticks a lane may wait for work
+    unsigned long long backoff_ticks; // persistent form: idle ticks after which a lane polls less often
     unsigned long long chunk_nodes; // persistent form: target nodes per fetched chunk of items
     unsigned long long n_lanes;     // persistent form: lanes of the launch (grid * workgroup)
     // serving
@@ -107,6 +136,8 @@ struct SearchArgs {
     ServeDone* done;                     // pinned host ring
     unsigned long long done_size;
     const unsigned int* host_stop;       // pinned host: 1 = stop
+    unsigned long long mb_poll_ticks;    // device-clock ticks between two reads of the mailbox tail
+    SplitRec* splits;                    // persistent form: SPLIT_RING donated subtrees
     // statistics: [0] probes, [1] nodes
     unsigned long long* stats;
 };

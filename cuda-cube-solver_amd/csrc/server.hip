@@ -50,6 +50,7 @@ struct ccs_server {
     QueueCtl* d_q = nullptr;
     unsigned long long* d_ring = nullptr;
     unsigned long long* d_stats = nullptr;
+    SplitRec* d_splits = nullptr;
     ServeRec* mb = nullptr;                 // pinned
     unsigned long long* mb_tail = nullptr;  // pinned
     ServeDone* done = nullptr;              // pinned
@@ -169,6 +170,8 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     HIP_TRY(hipMalloc(&s->d_q, sizeof(QueueCtl)));
     HIP_TRY(hipMalloc(&s->d_ring, sizeof(unsigned long long) * ring_size));
     HIP_TRY(hipMalloc(&s->d_stats, 4 * sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc(&s->d_splits, sizeof(SplitRec) * SPLIT_RING));
+    HIP_TRY(hipMemset(s->d_splits, 0, sizeof(SplitRec) * SPLIT_RING));
     const unsigned flags = hipHostMallocCoherent | hipHostMallocMapped;
     HIP_TRY(hipHostMalloc(&s->mb, sizeof(ServeRec) * s->mb_size, flags));
     HIP_TRY(hipHostMalloc(&s->mb_tail, 64, flags));
@@ -218,6 +221,15 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     A.poll_mask = 63;  // found flag refreshed every 64 picks (read one pick later)
     A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : SERVE_CHUNK_NODES;
     A.idle_limit = ~0ull;  // idling is normal while serving
+    {
+        int rate_khz = 0, dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+        const unsigned long long per_us = (unsigned long long)std::max(rate_khz, 1000) / 1000ull;
+        A.mb_poll_ticks = 2ull * per_us;   // one read of the host-side mailbox tail per 2 us
+        A.backoff_ticks = 20ull * per_us;
+    }
+    A.splits = s->d_splits;
     A.serving = 1;
     A.mb = s->mb;
     A.mb_tail = s->mb_tail;
@@ -277,6 +289,7 @@ int ccs_server_stop(ccs_server_t* s) {
     (void)hipFree(s->d_jobs);
     (void)hipFree(s->d_q);
     (void)hipFree(s->d_ring);
+    (void)hipFree(s->d_splits);
     (void)hipFree(s->d_stats);
     (void)hipHostFree(s->mb);
     (void)hipHostFree(s->mb_tail);

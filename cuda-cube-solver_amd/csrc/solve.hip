@@ -264,9 +264,10 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             lj[k] = jobs[live[k]];
             uint32_t b = lj[k].bound;
             uint32_t plen = std::min(std::min(b, (uint32_t)o.prefix_len), (uint32_t)MAX_PLEN);
-            // first round: chunks of up to 64 items, and at least max(4096, the
-            // cube's share of the lanes) chunks per round
-            const unsigned long long want = std::max<unsigned long long>(4096ull, A.n_lanes / (unsigned long long)nl);
+            // first round: chunks of up to 64 items, and at least max(4096,
+            // CHUNKS_PER_LANE per lane of the cube's share of the lanes) chunks
+            const unsigned long long want =
+                std::max<unsigned long long>(4096ull, CHUNKS_PER_LANE * A.n_lanes / (unsigned long long)nl);
             unsigned long long clog = 0;
             while (clog < 6 && (unsigned long long)(t->prefix_n[plen] >> (clog + 1)) >= want) ++clog;
             lj[k].ticket = ((unsigned long long)b << 49) | ((unsigned long long)plen << 54) | (clog << 59);
@@ -285,7 +286,10 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
         HIP_TRY(buf.alloc(&d_q, sizeof(QueueCtl)));
         HIP_TRY(buf.alloc(&d_ring, sizeof(unsigned long long) * ring_size));
         HIP_TRY(buf.alloc(&d_clock, sizeof(unsigned long long)));
+        SplitRec* d_splits = nullptr;
+        HIP_TRY(buf.alloc(&d_splits, sizeof(SplitRec) * SPLIT_RING));
         if (nl) {
+            HIP_TRY(hipMemsetAsync(d_splits, 0, sizeof(SplitRec) * SPLIT_RING, t->stream));
             HIP_TRY(hipMemcpyAsync(d_jobs, lj.data(), sizeof(SearchJob) * (size_t)nl, hipMemcpyHostToDevice, t->stream));
             HIP_TRY(hipMemcpyAsync(d_ring, ring.data(), sizeof(unsigned long long) * ring_size, hipMemcpyHostToDevice,
                                    t->stream));
@@ -297,6 +301,8 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             HIP_TRY(hipGetDevice(&dev0));
             HIP_TRY(hipDeviceGetAttribute(&rate_khz0, hipDeviceAttributeWallClockRate, dev0));
             A.idle_limit = (unsigned long long)std::max(rate_khz0, 1) * 60000ull;  // 60 s
+            A.backoff_ticks = (unsigned long long)std::max(rate_khz0, 1) / 50ull;  // 20 us
+            A.splits = d_splits;
             hipLaunchKernelGGL(read_clock_kernel, dim3(1), dim3(1), 0, t->stream, d_clock);
             HIP_TRY(hipEventRecord(ev0, t->stream));
             HIP_TRY(launch_search(t->K, t->format, 1, A, grid, o.block, t->stream));
