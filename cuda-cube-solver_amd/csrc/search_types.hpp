@@ -17,10 +17,6 @@ constexpr unsigned long long DEFAULT_CHUNK_NODES = 262144;
 // from waiting seconds on a single lane.
 constexpr unsigned long long SERVE_CHUNK_NODES = 16384;
 constexpr int MAX_PLEN = 7;     // longest work-item prefix
-// Persistent form: a round is cut into at least this many chunks per lane of
-// its cube's share of the launch (a lone cube then takes 1-2 items per fetch
-// at 7.7M items).
-constexpr unsigned long long CHUNKS_PER_LANE = 16;
 
 // One cube being searched, resident in HBM (the role of the reference's
 // CubeJob, kernels.cu:702-710, plus the per-cube round state of the

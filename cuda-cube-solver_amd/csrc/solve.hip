@@ -264,10 +264,9 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             lj[k] = jobs[live[k]];
             uint32_t b = lj[k].bound;
             uint32_t plen = std::min(std::min(b, (uint32_t)o.prefix_len), (uint32_t)MAX_PLEN);
-            // first round: chunks of up to 64 items, and at least max(4096,
-            // CHUNKS_PER_LANE per lane of the cube's share of the lanes) chunks
-            const unsigned long long want =
-                std::max<unsigned long long>(4096ull, CHUNKS_PER_LANE * A.n_lanes / (unsigned long long)nl);
+            // first round: chunks of up to 64 items, and at least max(4096, the
+            // cube's share of the lanes) chunks per round
+            const unsigned long long want = std::max<unsigned long long>(4096ull, A.n_lanes / (unsigned long long)nl);
             unsigned long long clog = 0;
             while (clog < 6 && (unsigned long long)(t->prefix_n[plen] >> (clog + 1)) >= want) ++clog;
             lj[k].ticket = ((unsigned long long)b << 49) | ((unsigned long long)plen << 54) | (clog << 59);

@@ -113,6 +113,21 @@ def test_d4t_distance_20_node_counts(td4t, reference_results):
         assert sum(_complete_nodes(s) for s in sols) == want
 
 
+def test_superflip_alone_batched_d4t(td4t):
+    """The superflip alone in the persistent form: its last rounds run on few
+    long items, which lanes out of work split among themselves (donated
+    subtrees).  Length 20, and the completed rounds (bounds 9..19) hold exactly
+    the nodes of the undivided search: 36,871,622,154, the figure the
+    persistent form gave before subtrees were donated (per round 43,254 ...
+    31,273,420,374)."""
+    sf = states.flip_heavy_states(2, 1)[-1]
+    assert sf.eo == (1,) * 12
+    sol = GpuSolver(table=td4t, mode="batched").solve([sf])[0]
+    assert len(sol.moves) == 20 and _solves([sf], [sol])
+    assert [b for b, _ in sol.iteration_nodes][:11] == list(range(9, 20))
+    assert _complete_nodes(sol) == 36_871_622_154
+
+
 def test_c2_seed2_on_d4t(td4t, seeded):
     """1,000 seed-2 states: the batched form equals the plain form (lengths and
     completed-round nodes per cube); the first 64 equal the oracle's."""
