@@ -820,17 +820,18 @@ __device__ __forceinline__ void end_chunk(const SearchArgs& A, uint32_t c, Searc
     release_items(A, c, 1);
 }
 
-// Donated subtrees.  Once no cube has items left to hand out, lanes that find
-// no work raise split_want; every SPLIT_CHECK passes each wave looks at it, and
-// a lane that has spent SPLIT_MIN_NODES nodes on its item (so the item is a
-// long one) hands the untried moves of its shallowest level, if they are at
-// least SPLIT_MIN_LEFT moves from the bound, to a SplitRec: that level is then
-// done for it, so its item root moves below it.  Donating short items as well
-// costs more in replays (and their probes) than it gains.  The taker replays the path
-// (its moves are not counted again: node counts stay those of an undivided
-// round) and searches the node from the move after the donor's one.  A record
-// holds one in-flight share of the round until its taker releases it, so a
-// round still closes only when all of its work has drained.
+// Donated subtrees (batches).  Once no cube has items left to hand out and
+// lanes have waited a while for work, they raise split_want; every
+// SPLIT_CHECK passes each wave looks at it, and a lane that has spent
+// SPLIT_MIN_NODES nodes on its item (so the item is a long one) hands the
+// untried moves of its shallowest level, if they are at least SPLIT_MIN_LEFT
+// moves from the bound, to a SplitRec: that level is then done for it, so its
+// item root moves below it.  Donating short items as well costs more in
+// replays (and their probes) than it gains.  The taker replays the path (its
+// moves are not counted again: node counts stay those of an undivided round)
+// and searches the node from the move after the donor's one.  A record holds
+// one in-flight share of the round until its taker releases it, so a round
+// still closes only when all of its work has drained.
 constexpr uint32_t SPLIT_CHECK = 512;  // passes between two looks of a wave (a power of two)
 constexpr int SPLIT_MIN_LEFT = 7;
 constexpr unsigned long long SPLIT_MIN_NODES = 2048;
@@ -1006,7 +1007,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     bool done = false;
 
     for (;;) {
-        if (QUEUE) {
+        if (QUEUE && !SERVE) {
             // Donation: every SPLIT_CHECK passes the wave looks at whether lanes are
             // out of work (one load, a wait of about a microsecond in some
             // milliseconds of passes); if so, each lane deep in a subtree hands the
@@ -1184,12 +1185,16 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                                 (SERVE && sys_load(A.host_stop))) {
                                 done = true;
                             } else {
-                                // out of work: ask for donations, and back off, longer once
-                                // the wait has lasted (idle waves then poll the queue words
-                                // far less often)
-                                if (rank == 0 && !ld_c(&A.q->split_want)) st_c(&A.q->split_want, 1u);
+                                // out of work: once the wait has lasted, ask for donations
+                                // (batches only: while serving, donated work waits behind new
+                                // rounds in the ring and holds its own round open, which cost
+                                // 13% of the served rate at 80% load) and back off longer, so
+                                // that idle waves poll the queue words far less
                                 unsigned long long now = wall_clock64();
-                                idle = idle_since && now - idle_since > A.backoff_ticks ? 1024 : 64;
+                                const bool waited = idle_since && now - idle_since > A.backoff_ticks;
+                                if (!SERVE && waited && rank == 0 && !ld_c(&A.q->split_want))
+                                    st_c(&A.q->split_want, 1u);
+                                idle = waited ? 1024 : 64;
                                 // watchdog: a lane that finds no work for idle_limit ends the launch
                                 if (!idle_since) idle_since = now;
                                 else if (now - idle_since > A.idle_limit) atomicMax(&A.q->stop, 2u);
