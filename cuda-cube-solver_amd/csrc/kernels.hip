@@ -1052,7 +1052,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         todo = 0;
                     } else {
                         todo = L.amask[m + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g));
-                        ord = frame_order(hp, bound - (uint32_t)g);
+                        ord = 0;  // recomputed before its next use
                     }
                     if (phase == 2 && g == root) {
                         // the prefix is in place; a donated subtree resumes after the
@@ -1279,7 +1279,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
                     }
                     g = 0;
-                    ord = frame_order(hp, bound);
+                    ord = 0;
                 }
                 // moves shared with the held state count as replayed (node counts are
                 // those of a replay from the root); a donated subtree's path was
@@ -1300,9 +1300,14 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 hp = hst[g * nt];
                 const int last = g > 0 ? (int)pth[(g - 1) * nt] : -1;
                 todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g)) & ~((2u << pm) - 1u);
-                ord = frame_order(hp, bound - (uint32_t)g);
+                ord = 0;
                 if (phase == 1 && g == tgt) phase = 2;
             }
+            // The frame order of the lane's node, after a push, a pop or a new
+            // item's root (which mark it stale with 0; an actual order has one
+            // bit per frame): computed here once for all three, so that a wave
+            // whose lanes did different ones runs it once.
+            if (ord == 0u) ord = frame_order(hp, bound - (uint32_t)g);
             if (phase == 2) {
                 // ---- next prefix move, evaluated like a child ----
                 m = (int)((pbits >> (8 * g)) & 0xFFu);
