@@ -4,7 +4,9 @@ One step = one batched solve of the workload (4,096 seeded uniform random
 states, seed 3, on d4t, one persistent launch).  Before it: the d4t tables
 (block and base-3) are built from scratch on the GPU, reported per layer as
 `table_build`, and the autotuner picks the persistent form's knobs on a
-separate 512-state sample (seed 33); neither is part of the step.  After it:
+separate 2,048-state sample (seed 33: large enough that its final rounds are
+ordered by item keys as the workload's are); neither is part of the step.
+After it:
 the first 128 states solved one at a time in the plain form, the random-probe
 ceiling of the memory system, a Poisson serving run at 50% and 80% of the
 measured capacity (seed 5), and the CPU oracle on a bounded sample.  Prints
@@ -134,7 +136,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-probe ceiling measurement")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use --chunk-nodes etc.")
-    ap.add_argument("--tune-cubes", type=int, default=512, help="autotuner sample (seeded uniform states, seed 33)")
+    ap.add_argument("--tune-cubes", type=int, default=2048, help="autotuner sample (seeded uniform states, seed 33)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--prefix-len", type=int, default=4)
     ap.add_argument("--chunk-nodes", type=int, default=0, help="persistent form: nodes per fetched chunk (0: library default)")

@@ -569,6 +569,7 @@ constexpr int T_PLEN_SHIFT = 54;
 constexpr unsigned long long T_CLOSING = 1ull << 57;
 constexpr unsigned long long T_RETIRED = 1ull << 58;  // serving: the cube's slot is free again
 constexpr int T_CHUNK_SHIFT = 59;
+constexpr unsigned long long T_BANDED = 1ull << 62;  // the round visits its items in N_BANDS key bands
 // Serving: a fetching wave admits new cubes while fewer rounds than this are queued.
 constexpr unsigned long long ADMIT_BELOW = 256;
 // Ring entry: (slot mod 2^27) << 37 | round bound << 32 | cube.  The slot part
@@ -591,6 +592,8 @@ __device__ __forceinline__ void ring_push(const SearchArgs& A, uint32_t bound, u
 // turned up, half the round on average.  Consecutive items within a block
 // still share their leading prefix moves.
 constexpr uint32_t ITEM_BLOCK = 8;
+constexpr uint32_t CL_BANDED = 1u << 16;  // chunk_left: the round is banded (visits, not items)
+constexpr uint32_t CL_KEYS = 1u << 17;    // chunk_left: record the items' keys
 __device__ __forceinline__ uint32_t next_item(uint32_t item, uint32_t left, uint32_t lg, uint32_t n) {
     const uint32_t blk = min(ITEM_BLOCK, 1u << lg);
     if (left & (blk - 1u)) return item + 1u;  // within a block
@@ -604,9 +607,28 @@ __device__ __forceinline__ uint32_t t_bound(unsigned long long t) { return (uint
 __device__ __forceinline__ uint32_t t_plen(unsigned long long t) { return (uint32_t)(t >> T_PLEN_SHIFT) & 7u; }
 __device__ __forceinline__ uint32_t t_chunk(unsigned long long t) { return (uint32_t)(t >> T_CHUNK_SHIFT) & 7u; }
 __device__ __forceinline__ unsigned long long make_ticket(unsigned long long keep, uint32_t bound, uint32_t plen,
-                                                          uint32_t chunk_log) {
+                                                          uint32_t chunk_log, bool banded = false) {
     return (keep & T_INF_MASK) | ((unsigned long long)bound << T_BOUND_SHIFT) |
-           ((unsigned long long)plen << T_PLEN_SHIFT) | ((unsigned long long)chunk_log << T_CHUNK_SHIFT);
+           ((unsigned long long)plen << T_PLEN_SHIFT) | ((unsigned long long)chunk_log << T_CHUNK_SHIFT) |
+           (banded ? T_BANDED : 0ull);
+}
+// Fetch slots of a round: its items; when it is banded, each band's items
+// rounded up to whole chunks, once per band.  Visit v of a banded round is item
+// v mod n of band v / n (an item is searched in the visit of its band); the
+// chunks of a band are laid out over its items as a plain round's are.
+__device__ __forceinline__ uint32_t t_visits(const SearchArgs& A, unsigned long long t) {
+    const uint32_t n = A.pre_n[t_plen(t)], lg = t_chunk(t);
+    return (t & T_BANDED) ? N_BANDS * (((n + (1u << lg) - 1u) >> lg) << lg) : n;
+}
+// Band of an item key (0: unknown, searched last) under a cube's bounds.
+__device__ __forceinline__ uint32_t key_band(uint32_t key, uint32_t thr0, uint32_t thr1) {
+    if (key == 0u) return N_BANDS - 1u;
+    uint32_t b = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b += ((thr0 >> (8 * i)) & 0xFFu) < key;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) b += ((thr1 >> (8 * i)) & 0xFFu) < key;
+    return b;
 }
 // Items per fetch for a round: about chunk_nodes nodes per chunk, estimated from
 // the previous round's nodes and the usual growth of ~13x per bound, but at
@@ -648,13 +670,40 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
     uint32_t nb = bound + 1;
     uint32_t p = (uint32_t)A.prefix_len + (nodes >= A.deeper0) + (nodes >= A.deeper1);
     uint32_t plen = min(min(nb, p), (uint32_t)MAX_PLEN);
-    uint32_t clog = chunk_log_for(A, nodes, A.pre_n[plen]);
+    // Item order of the next round.  The key of an item (the bounds at its
+    // prefix's end) does not depend on the round's bound, so the keys this
+    // round recorded order the next one when the prefix length stays: items
+    // with lower bounds at their prefix's end hold a solution more often, and
+    // going through them first cuts the final round's nodes.  Bands of equal
+    // sampled counts stand in for a sort; only long rounds are banded, where
+    // the visits of other bands' items cost little.
+    bool banded = false;
+    if (A.key_hist) {
+        uint32_t* hist = A.key_hist + (size_t)c * KEY_BINS;
+        if (plen == t_plen(tk) && A.keys[plen] && nodes >= BAND_MIN_NODES_PER_ITEM * A.pre_n[plen]) {
+            uint32_t total = 0;
+            for (uint32_t v = 1; v < KEY_BINS; ++v) total += ld_c(&hist[v]);
+            if (total >= 64u) {
+                uint32_t thr[N_BANDS - 1], acc = 0, b = 0;
+                for (uint32_t v = 1; v < KEY_BINS && b < N_BANDS - 1; ++v) {
+                    acc += ld_c(&hist[v]);
+                    while (b < N_BANDS - 1 && (unsigned long long)acc * N_BANDS >= (unsigned long long)total * (b + 1)) thr[b++] = v;
+                }
+                while (b < N_BANDS - 1) thr[b++] = 255u;
+                st_c(&J->band_thr[0], thr[0] | (thr[1] << 8) | (thr[2] << 16) | (thr[3] << 24));
+                st_c(&J->band_thr[1], thr[4] | (thr[5] << 8) | (thr[6] << 16) | (255u << 24));
+                banded = true;
+            }
+        }
+        for (uint32_t v = 0; v < KEY_BINS; ++v) st_c(&hist[v], 0u);
+    }
+    uint32_t clog = chunk_log_for(A, nodes, A.pre_n[plen] * (banded ? N_BANDS : 1u));
     st_c(&J->nodes, 0ull);
     st_c(&J->bound, nb);
     complete();
     unsigned long long cur = ld_c(&J->ticket);
     for (;;) {  // keep the in-flight count of late fetchers, reset the item cursor
-        unsigned long long prev = atomicCAS(&J->ticket, cur, make_ticket(cur, nb, plen, clog));
+        unsigned long long prev = atomicCAS(&J->ticket, cur, make_ticket(cur, nb, plen, clog, banded));
         if (prev == cur) break;
         cur = prev;
     }
@@ -663,7 +712,7 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
 }
 
 __device__ __forceinline__ bool round_drained(const SearchArgs& A, unsigned long long t) {
-    return t_inflight(t) == 0 && !(t & (T_CLOSING | T_RETIRED)) && t_next(t) >= A.pre_n[t_plen(t)];
+    return t_inflight(t) == 0 && !(t & (T_CLOSING | T_RETIRED)) && t_next(t) >= t_visits(A, t);
 }
 
 template <typename T>
@@ -996,7 +1045,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // read one item ahead so that no item start waits on a global load.
     unsigned long long pnext = 0;
     uint32_t found_seen = 0;
-    // persistent form: the lane's chunk, items left (low byte) | log2 chunk size << 8
+    // persistent form: the lane's chunk: items (visits) left in the low byte,
+    // log2 chunk size << 8, CL_BANDED, CL_KEYS
     uint32_t chunk_left = 0, item_cur = 0;
     // The lane's frames stay at depth `hold` on the previous item's prefix
     // (-1: nothing held); `hfail` is the depth of that item's failing prefix
@@ -1059,6 +1109,18 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         // donor's move, whose successor is in the byte past its path
                         phase = 3;
                         todo &= ~((1u << (uint32_t)((pbits >> (8 * g)) & 0xFFu)) - 1u);
+                        if (QUEUE && (chunk_left & CL_KEYS)) {  // the item's key, for the next round's order
+                            const uint32_t n_items = A.pre_n[plen];
+                            const uint32_t pi = (chunk_left & CL_BANDED) ? item_cur % n_items : item_cur;
+                            uint32_t key = 1u;
+#pragma unroll
+                            for (int j = 0; j < 6; ++j) key += hget(hp, j);
+                            // this round's buffer: the round reads only the other one, so an
+                            // item's key never changes under the round's own visits
+                            atomicOr(&A.keys[plen][((size_t)c * 2u + (bound & 1u)) * A.key_words[plen] + (pi >> 2)],
+                                     key << (8u * (pi & 3u)));
+                            if ((pi & 7u) == 0u) atomicAdd(&A.key_hist[(size_t)c * KEY_BINS + key], 1u);
+                        }
                     }
                 } else if (phase == 2) {  // a prefix move was pruned: the item ends
                     hfail = g;
@@ -1119,10 +1181,12 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 } else if (chunk_left & 0xFFu) {
                     // next item of this lane's chunk: same cube and round, no atomics
                     const uint32_t n_items = A.pre_n[plen];
-                    item = next_item(item_cur, chunk_left & 0xFFu, chunk_left >> 8, n_items);
+                    const uint32_t bbase = (chunk_left & CL_BANDED) ? (item_cur / n_items) * n_items : 0u;
+                    const uint32_t w = next_item(item_cur - bbase, chunk_left & 0xFFu, (chunk_left >> 8) & 7u, n_items);
+                    item = bbase + w;
                     --chunk_left;
                     fresh = false;
-                    if (item >= n_items) {  // the rest of the chunk lies past the round's last item
+                    if (w >= n_items) {  // the rest of the chunk lies past the round's (band's) last item
                         chunk_left = 0;
                         end_chunk(A, c, J, chunk_nodes);
                         continue;
@@ -1157,7 +1221,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                                 SearchJob* Je = &A.jobs[ce];
                                 unsigned long long tk = ld_c(&Je->ticket);
                                 if ((tk & (T_CLOSING | T_RETIRED)) || t_bound(tk) != eb ||
-                                    t_next(tk) >= A.pre_n[t_plen(tk)] || ld_c(&Je->found)) {
+                                    t_next(tk) >= t_visits(A, tk) || ld_c(&Je->found)) {
                                     atomicCAS(&A.q->head, h, h + 1);  // stale, finished or fully handed out
                                     continue;
                                 }
@@ -1169,7 +1233,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                                 // sequential search would).  Only the wave that advances the head
                                 // re-queues the entry, and only while items remain.
                                 if (atomicCAS(&A.q->head, h, h + 1) == h &&
-                                    t_next(old) + ((unsigned long long)cnt << clog) < A.pre_n[t_plen(old)])
+                                    t_next(old) + ((unsigned long long)cnt << clog) < t_visits(A, old))
                                     ring_push(A, eb, ce);
                                 cc = ce;
                                 break;
@@ -1217,9 +1281,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             chunk_nodes = 0;
                         } else {
                             const uint32_t pl = t_plen(old);
-                            const uint32_t n_items = A.pre_n[pl];
                             item = t_next(old) + (rank << clog);
-                            bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < n_items;
+                            bool valid = !(old & (T_CLOSING | T_RETIRED)) && item < t_visits(A, old);
                             unsigned long long inval = __ballot(!valid);
                             if (rank == 0 && inval) release_items(A, cc, __popcll(inval));
                             if (!valid) continue;
@@ -1227,11 +1290,18 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             J = &A.jobs[c];
                             bound = t_bound(old);
                             plen = pl;
-                            item = (item >> clog) * min(ITEM_BLOCK, 1u << clog);  // chunk's first item
+                            if (old & T_BANDED) {  // chunk ci of the round is chunk ci mod Q of band ci / Q
+                                const uint32_t n_items = A.pre_n[pl], nq = (n_items + (1u << clog) - 1u) >> clog;
+                                const uint32_t ci = item >> clog, band = ci / nq;
+                                item = band * n_items + (ci - band * nq) * min(ITEM_BLOCK, 1u << clog);
+                            } else {
+                                item = (item >> clog) * min(ITEM_BLOCK, 1u << clog);  // chunk's first item
+                            }
                             item_cur = item;
                             fresh = true;  // a chunk may follow a reuse of the slot: start from the root
                             hold = -1;
-                            chunk_left = ((1u << clog) - 1u) | (clog << 8);
+                            chunk_left = ((1u << clog) - 1u) | (clog << 8) | ((old & T_BANDED) ? CL_BANDED : 0u) |
+                                         (!SERVE && A.keys[pl] ? CL_KEYS : 0u);
                             chunk_nodes = 0;
                             if (ld_c(&J->found)) {
                                 chunk_left = 0;
@@ -1245,10 +1315,44 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 // the probe step below, all six frames (kernels.cu:1552-1614) ----
                 unsigned long long nb;
                 if (QUEUE) {
-                    nb = spl ? spath : fresh ? A.pre_tab[plen][item] : pnext;
-                    if (chunk_left & 0xFFu) {  // the next item's prefix, used when it starts
+                    const bool banded = !spl && (chunk_left & CL_BANDED);
+                    uint32_t pitem = item;
+                    if (banded) {
+                        // visit `item` is item pitem of band `band`: searched only there,
+                        // by the keys of the previous round (its buffer: bound - 1)
                         const uint32_t n_items = A.pre_n[plen];
-                        const uint32_t nx = next_item(item, chunk_left & 0xFFu, chunk_left >> 8, n_items);
+                        const uint32_t band = item / n_items;
+                        pitem = item - band * n_items;
+                        const unsigned long long* krow = reinterpret_cast<const unsigned long long*>(
+                            A.keys[plen] + ((size_t)c * 2u + ((bound - 1u) & 1u)) * A.key_words[plen]);
+                        const unsigned long long kw = ld_c(&krow[pitem >> 3]);  // keys of 8 items
+                        const uint32_t t0 = ld_c(&J->band_thr[0]), t1 = ld_c(&J->band_thr[1]);
+                        bool member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, t0, t1) == band;
+                        // the chunk's next visits whose keys came with the same load (the
+                        // rest of its block of ITEM_BLOCK consecutive items)
+                        const uint32_t blk = min(ITEM_BLOCK, 1u << ((chunk_left >> 8) & 7u));
+                        while (!member && (chunk_left & 0xFFu & (blk - 1u)) && ((pitem + 1u) & 7u) && pitem + 1u < n_items) {
+                            ++item;
+                            ++pitem;
+                            --chunk_left;
+                            item_cur = item;
+                            member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, t0, t1) == band;
+                        }
+                        if (!member) {
+                            if (!(chunk_left & 0xFFu)) {
+                                chunk_left = 0;
+                                end_chunk(A, c, J, chunk_nodes);
+                            }
+                            // nothing searched and nothing held changes; the pass ends
+                            // here, so that a run of other bands' visits does not
+                            // hold the whole wave
+                            break;
+                        }
+                    }
+                    nb = spl ? spath : (fresh || banded) ? A.pre_tab[plen][pitem] : pnext;
+                    if ((chunk_left & 0xFFu) && !banded) {  // the next item's prefix, used when it starts
+                        const uint32_t n_items = A.pre_n[plen];
+                        const uint32_t nx = next_item(item, chunk_left & 0xFFu, (chunk_left >> 8) & 7u, n_items);
                         if (nx < n_items) pnext = A.pre_tab[plen][nx];
                     }
                     found_seen = ld_c(&J->found);  // looked at when this item ends

@@ -287,6 +287,24 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
         HIP_TRY(buf.alloc(&d_clock, sizeof(unsigned long long)));
         SplitRec* d_splits = nullptr;
         HIP_TRY(buf.alloc(&d_splits, sizeof(SplitRec) * SPLIT_RING));
+        // item keys (two buffers per cube and prefix length) and key histograms
+        uint32_t *d_keys = nullptr, *d_hist = nullptr;
+        size_t key_total = 0;
+        // (rows of whole 8-byte words: the search reads eight keys at a time)
+        for (int p = o.prefix_len; p <= KEY_MAX_PLEN; ++p) key_total += 2ull * nl * ((t->prefix_n[p] + 7ull) / 8ull * 2ull);
+        if (nl >= BAND_MIN_CUBES && key_total) {
+            HIP_TRY(buf.alloc(&d_keys, key_total * 4));
+            HIP_TRY(buf.alloc(&d_hist, (size_t)nl * KEY_BINS * 4));
+            size_t off = 0;
+            for (int p = o.prefix_len; p <= KEY_MAX_PLEN; ++p) {
+                A.keys[p] = d_keys + off;
+                A.key_words[p] = (t->prefix_n[p] + 7ull) / 8ull * 2ull;
+                off += 2ull * nl * A.key_words[p];
+            }
+            A.key_hist = d_hist;
+            HIP_TRY(hipMemsetAsync(d_keys, 0, key_total * 4, t->stream));
+            HIP_TRY(hipMemsetAsync(d_hist, 0, (size_t)nl * KEY_BINS * 4, t->stream));
+        }
         if (nl) {
             HIP_TRY(hipMemsetAsync(d_splits, 0, sizeof(SplitRec) * SPLIT_RING, t->stream));
             HIP_TRY(hipMemcpyAsync(d_jobs, lj.data(), sizeof(SearchJob) * (size_t)nl, hipMemcpyHostToDevice, t->stream));

@@ -7,7 +7,7 @@ items get one and two moves longer).  Probes in flight per lane are fixed by
 the kernel (one forward and one inverse frame per step).  Every setting gives
 the same optimal lengths; the tuner checks that too.
 
-    python -m cuda_cube_solver_amd.tune --table d4t --cubes 512 --seed 33 [--out knobs.json]
+    python -m cuda_cube_solver_amd.tune --table d4t --cubes 2048 --seed 33 [--out knobs.json]
 """
 from __future__ import annotations
 
@@ -67,7 +67,7 @@ def knobs(best: dict) -> dict:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="cuda_cube_solver_amd.tune")
     ap.add_argument("--table", default="d4t")
-    ap.add_argument("--cubes", type=int, default=512)
+    ap.add_argument("--cubes", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=33)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)

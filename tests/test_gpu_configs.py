@@ -151,6 +151,20 @@ def test_c3_seed3_first64_on_d4t(td4t, seeded):
     assert [_complete_nodes(s) for s in sols] == fx["complete_round_nodes"]
 
 
+def test_c3_seed3_2048_banded_order(td4t, seeded):
+    """2,048 seed-3 states in one persistent launch, large enough that long
+    rounds visit their items in bands of the keys the previous round recorded:
+    the first 64 cubes still give the oracle's lengths and completed-round
+    nodes (every item searched exactly once per round), and every solution
+    solves."""
+    cubes = states.uniform_states(2048, 3)
+    sols = GpuSolver(table=td4t, mode="batched", prefix_len=4, deeper_prefix_at=(5e7, 3e9)).solve(cubes)
+    fx = seeded["c3_d4t_seed3_first64"]
+    assert [len(s.moves) for s in sols[:64]] == fx["lengths"]
+    assert [_complete_nodes(s) for s in sols[:64]] == fx["complete_round_nodes"]
+    assert _solves(cubes, sols)
+
+
 def test_c1_seed1_on_304_plain_one_at_a_time(seeded, reference_results):
     """C1: 64 seed-1 states on 304, each solved alone in the plain form (one
     launch per IDA* round): the oracle's lengths and completed-round nodes."""
