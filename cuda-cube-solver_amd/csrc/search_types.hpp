@@ -25,6 +25,8 @@ constexpr int MAX_PLEN = 7;     // longest work-item prefix
 // 10% faster.)
 constexpr int KEY_MAX_PLEN = 5;
 constexpr int BAND_MIN_CUBES = 2048;
+// ... and whose keys fit in this many bytes (1.24 MB per cube: about 13,500 cubes)
+constexpr unsigned long long KEY_MAX_BYTES = 16ull << 30;
 constexpr uint32_t N_BANDS = 8;
 constexpr uint32_t KEY_BINS = 128;
 constexpr unsigned long long BAND_MIN_NODES_PER_ITEM = 32;

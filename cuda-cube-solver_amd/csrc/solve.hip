@@ -292,7 +292,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
         size_t key_total = 0;
         // (rows of whole 8-byte words: the search reads eight keys at a time)
         for (int p = o.prefix_len; p <= KEY_MAX_PLEN; ++p) key_total += 2ull * nl * ((t->prefix_n[p] + 7ull) / 8ull * 2ull);
-        if (nl >= BAND_MIN_CUBES && key_total) {
+        if (nl >= BAND_MIN_CUBES && key_total && key_total * 4 <= KEY_MAX_BYTES) {
             HIP_TRY(buf.alloc(&d_keys, key_total * 4));
             HIP_TRY(buf.alloc(&d_hist, (size_t)nl * KEY_BINS * 4));
             size_t off = 0;
