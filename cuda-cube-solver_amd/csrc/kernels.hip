@@ -620,14 +620,13 @@ __device__ __forceinline__ uint32_t t_visits(const SearchArgs& A, unsigned long 
     const uint32_t n = A.pre_n[t_plen(t)], lg = t_chunk(t);
     return (t & T_BANDED) ? N_BANDS * (((n + (1u << lg) - 1u) >> lg) << lg) : n;
 }
-// Band of an item key (0: unknown, searched last) under a cube's bounds.
-__device__ __forceinline__ uint32_t key_band(uint32_t key, uint32_t thr0, uint32_t thr1) {
+// Band of an item key (0: unknown, searched last) under a cube's band bounds
+// (N_BANDS - 1 bytes, four to a word, the last byte 255).
+__device__ __forceinline__ uint32_t key_band(uint32_t key, const uint32_t (&thr)[N_BANDS / 4]) {
     if (key == 0u) return N_BANDS - 1u;
     uint32_t b = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) b += ((thr0 >> (8 * i)) & 0xFFu) < key;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) b += ((thr1 >> (8 * i)) & 0xFFu) < key;
+    for (uint32_t i = 0; i < N_BANDS - 1; ++i) b += ((thr[i / 4] >> (8 * (i % 4))) & 0xFFu) < key;
     return b;
 }
 // Items per fetch for a round: about chunk_nodes nodes per chunk, estimated from
@@ -690,8 +689,13 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
                     while (b < N_BANDS - 1 && (unsigned long long)acc * N_BANDS >= (unsigned long long)total * (b + 1)) thr[b++] = v;
                 }
                 while (b < N_BANDS - 1) thr[b++] = 255u;
-                st_c(&J->band_thr[0], thr[0] | (thr[1] << 8) | (thr[2] << 16) | (thr[3] << 24));
-                st_c(&J->band_thr[1], thr[4] | (thr[5] << 8) | (thr[6] << 16) | (255u << 24));
+#pragma unroll
+                for (uint32_t w = 0; w < N_BANDS / 4; ++w) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (uint32_t i = 0; i < 4; ++i) word |= (4 * w + i < N_BANDS - 1 ? thr[4 * w + i] : 255u) << (8 * i);
+                    st_c(&J->band_thr[w], word);
+                }
                 banded = true;
             }
         }
@@ -1326,8 +1330,10 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         const unsigned long long* krow = reinterpret_cast<const unsigned long long*>(
                             A.keys[plen] + ((size_t)c * 2u + ((bound - 1u) & 1u)) * A.key_words[plen]);
                         const unsigned long long kw = ld_c(&krow[pitem >> 3]);  // keys of 8 items
-                        const uint32_t t0 = ld_c(&J->band_thr[0]), t1 = ld_c(&J->band_thr[1]);
-                        bool member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, t0, t1) == band;
+                        uint32_t thr[N_BANDS / 4];
+#pragma unroll
+                        for (uint32_t w = 0; w < N_BANDS / 4; ++w) thr[w] = ld_c(&J->band_thr[w]);
+                        bool member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, thr) == band;
                         // the chunk's next visits whose keys came with the same load (the
                         // rest of its block of ITEM_BLOCK consecutive items)
                         const uint32_t blk = min(ITEM_BLOCK, 1u << ((chunk_left >> 8) & 7u));
@@ -1336,7 +1342,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             ++pitem;
                             --chunk_left;
                             item_cur = item;
-                            member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, t0, t1) == band;
+                            member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, thr) == band;
                         }
                         if (!member) {
                             if (!(chunk_left & 0xFFu)) {

@@ -27,7 +27,7 @@ constexpr int KEY_MAX_PLEN = 5;
 constexpr int BAND_MIN_CUBES = 2048;
 // ... and whose keys fit in this many bytes (1.24 MB per cube: about 13,500 cubes)
 constexpr unsigned long long KEY_MAX_BYTES = 16ull << 30;
-constexpr uint32_t N_BANDS = 8;
+constexpr uint32_t N_BANDS = 16;  // a multiple of 4, at most 16
 constexpr uint32_t KEY_BINS = 128;
 constexpr unsigned long long BAND_MIN_NODES_PER_ITEM = 32;
 
@@ -50,7 +50,7 @@ struct alignas(64) SearchJob {
     unsigned long long t_admit;  // serving: device clock at admission
     uint32_t round_bound[MAX_BOUND];
     unsigned long long round_nodes[MAX_BOUND];
-    uint32_t band_thr[2];        // persistent form: item-key bounds of the round's bands (7 bytes)
+    uint32_t band_thr[4];        // persistent form: item-key bounds of the round's bands (N_BANDS - 1 bytes)
 };
 
 // Control block of the persistent form.
