@@ -620,14 +620,11 @@ __device__ __forceinline__ uint32_t t_visits(const SearchArgs& A, unsigned long 
     const uint32_t n = A.pre_n[t_plen(t)], lg = t_chunk(t);
     return (t & T_BANDED) ? N_BANDS * (((n + (1u << lg) - 1u) >> lg) << lg) : n;
 }
-// Band of an item key (0: unknown, searched last) under a cube's band bounds
-// (N_BANDS - 1 bytes, four to a word, the last byte 255).
-__device__ __forceinline__ uint32_t key_band(uint32_t key, const uint32_t (&thr)[N_BANDS / 4]) {
-    if (key == 0u) return N_BANDS - 1u;
-    uint32_t b = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < N_BANDS - 1; ++i) b += ((thr[i / 4] >> (8 * (i % 4))) & 0xFFu) < key;
-    return b;
+// Whether an item key (0: unknown, searched in the last band) is in band b,
+// whose keys are those above lo and up to hi (the cube's band bounds b - 1
+// and b: N_BANDS - 1 non-decreasing bytes, four to a word).
+__device__ __forceinline__ bool in_band(uint32_t key, uint32_t b, uint32_t lo, uint32_t hi) {
+    return key == 0u ? b == N_BANDS - 1u : (lo < key && key <= hi);
 }
 // Items per fetch for a round: about chunk_nodes nodes per chunk, estimated from
 // the previous round's nodes and the usual growth of ~13x per bound, but at
@@ -646,6 +643,41 @@ __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned 
     uint32_t lg = 0;
     while (lg < 6 && (per_item << (lg + 1)) <= A.chunk_nodes && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
     return lg;
+}
+
+// A cube's band bounds for its next round from the key histogram of the round
+// that ends (if `want`: the round was long and the prefix length stays), and
+// the histogram cleared.  Out of line: it runs once per round, and its
+// working set stays out of the search loop's registers.
+__device__ __attribute__((noinline)) bool band_bounds(uint32_t* hist, SearchJob* J, bool want) {
+    bool banded = false;
+    if (want) {
+        uint32_t total = 0;
+        for (uint32_t v = 1; v < KEY_BINS; ++v) total += ld_c(&hist[v]);
+        if (total >= 64u) {
+            uint32_t acc = 0, b = 0, word = 0;
+            for (uint32_t v = 1; v < KEY_BINS && b < N_BANDS - 1; ++v) {
+                acc += ld_c(&hist[v]);
+                while (b < N_BANDS - 1 && (unsigned long long)acc * N_BANDS >= (unsigned long long)total * (b + 1)) {
+                    word |= v << (8u * (b & 3u));
+                    if ((++b & 3u) == 0u) {
+                        st_c(&J->band_thr[(b >> 2) - 1u], word);
+                        word = 0;
+                    }
+                }
+            }
+            for (; b < N_BANDS; ++b) {  // bounds never reached, and the pad byte
+                word |= 255u << (8u * (b & 3u));
+                if (((b + 1u) & 3u) == 0u) {
+                    st_c(&J->band_thr[b >> 2], word);
+                    word = 0;
+                }
+            }
+            banded = true;
+        }
+    }
+    for (uint32_t v = 0; v < KEY_BINS; ++v) st_c(&hist[v], 0u);
+    return banded;
 }
 
 __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long tk) {
@@ -677,30 +709,9 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
     // sampled counts stand in for a sort; only long rounds are banded, where
     // the visits of other bands' items cost little.
     bool banded = false;
-    if (A.key_hist) {
-        uint32_t* hist = A.key_hist + (size_t)c * KEY_BINS;
-        if (plen == t_plen(tk) && A.keys[plen] && nodes >= BAND_MIN_NODES_PER_ITEM * A.pre_n[plen]) {
-            uint32_t total = 0;
-            for (uint32_t v = 1; v < KEY_BINS; ++v) total += ld_c(&hist[v]);
-            if (total >= 64u) {
-                uint32_t thr[N_BANDS - 1], acc = 0, b = 0;
-                for (uint32_t v = 1; v < KEY_BINS && b < N_BANDS - 1; ++v) {
-                    acc += ld_c(&hist[v]);
-                    while (b < N_BANDS - 1 && (unsigned long long)acc * N_BANDS >= (unsigned long long)total * (b + 1)) thr[b++] = v;
-                }
-                while (b < N_BANDS - 1) thr[b++] = 255u;
-#pragma unroll
-                for (uint32_t w = 0; w < N_BANDS / 4; ++w) {
-                    uint32_t word = 0;
-#pragma unroll
-                    for (uint32_t i = 0; i < 4; ++i) word |= (4 * w + i < N_BANDS - 1 ? thr[4 * w + i] : 255u) << (8 * i);
-                    st_c(&J->band_thr[w], word);
-                }
-                banded = true;
-            }
-        }
-        for (uint32_t v = 0; v < KEY_BINS; ++v) st_c(&hist[v], 0u);
-    }
+    if (A.key_hist)
+        banded = band_bounds(A.key_hist + (size_t)c * KEY_BINS, J,
+                             plen == t_plen(tk) && A.keys[plen] && nodes >= BAND_MIN_NODES_PER_ITEM * A.pre_n[plen]);
     uint32_t clog = chunk_log_for(A, nodes, A.pre_n[plen] * (banded ? N_BANDS : 1u));
     st_c(&J->nodes, 0ull);
     st_c(&J->bound, nb);
@@ -1113,7 +1124,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         // donor's move, whose successor is in the byte past its path
                         phase = 3;
                         todo &= ~((1u << (uint32_t)((pbits >> (8 * g)) & 0xFFu)) - 1u);
-                        if (QUEUE && (chunk_left & CL_KEYS)) {  // the item's key, for the next round's order
+                        if (QUEUE && !SERVE && (chunk_left & CL_KEYS)) {  // the item's key, for the next round's order
                             const uint32_t n_items = A.pre_n[plen];
                             const uint32_t pi = (chunk_left & CL_BANDED) ? item_cur % n_items : item_cur;
                             uint32_t key = 1u;
@@ -1185,7 +1196,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 } else if (chunk_left & 0xFFu) {
                     // next item of this lane's chunk: same cube and round, no atomics
                     const uint32_t n_items = A.pre_n[plen];
-                    const uint32_t bbase = (chunk_left & CL_BANDED) ? (item_cur / n_items) * n_items : 0u;
+                    const uint32_t bbase = (!SERVE && (chunk_left & CL_BANDED)) ? (item_cur / n_items) * n_items : 0u;
                     const uint32_t w = next_item(item_cur - bbase, chunk_left & 0xFFu, (chunk_left >> 8) & 7u, n_items);
                     item = bbase + w;
                     --chunk_left;
@@ -1294,7 +1305,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             J = &A.jobs[c];
                             bound = t_bound(old);
                             plen = pl;
-                            if (old & T_BANDED) {  // chunk ci of the round is chunk ci mod Q of band ci / Q
+                            if (!SERVE && (old & T_BANDED)) {  // chunk ci of the round is chunk ci mod Q of band ci / Q
                                 const uint32_t n_items = A.pre_n[pl], nq = (n_items + (1u << clog) - 1u) >> clog;
                                 const uint32_t ci = item >> clog, band = ci / nq;
                                 item = band * n_items + (ci - band * nq) * min(ITEM_BLOCK, 1u << clog);
@@ -1319,7 +1330,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 // the probe step below, all six frames (kernels.cu:1552-1614) ----
                 unsigned long long nb;
                 if (QUEUE) {
-                    const bool banded = !spl && (chunk_left & CL_BANDED);
+                    const bool banded = !SERVE && !spl && (chunk_left & CL_BANDED);
                     uint32_t pitem = item;
                     if (banded) {
                         // visit `item` is item pitem of band `band`: searched only there,
@@ -1330,10 +1341,10 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         const unsigned long long* krow = reinterpret_cast<const unsigned long long*>(
                             A.keys[plen] + ((size_t)c * 2u + ((bound - 1u) & 1u)) * A.key_words[plen]);
                         const unsigned long long kw = ld_c(&krow[pitem >> 3]);  // keys of 8 items
-                        uint32_t thr[N_BANDS / 4];
-#pragma unroll
-                        for (uint32_t w = 0; w < N_BANDS / 4; ++w) thr[w] = ld_c(&J->band_thr[w]);
-                        bool member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, thr) == band;
+                        const uint32_t lo = band ? (ld_c(&J->band_thr[(band - 1u) >> 2]) >> (8u * ((band - 1u) & 3u))) & 0xFFu : 0u;
+                        const uint32_t hi =
+                            band < N_BANDS - 1u ? (ld_c(&J->band_thr[band >> 2]) >> (8u * (band & 3u))) & 0xFFu : 255u;
+                        bool member = in_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, band, lo, hi);
                         // the chunk's next visits whose keys came with the same load (the
                         // rest of its block of ITEM_BLOCK consecutive items)
                         const uint32_t blk = min(ITEM_BLOCK, 1u << ((chunk_left >> 8) & 7u));
@@ -1342,7 +1353,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             ++pitem;
                             --chunk_left;
                             item_cur = item;
-                            member = key_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, thr) == band;
+                            member = in_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, band, lo, hi);
                         }
                         if (!member) {
                             if (!(chunk_left & 0xFFu)) {
