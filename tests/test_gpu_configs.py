@@ -165,6 +165,21 @@ def test_c3_seed3_2048_banded_order(td4t, seeded):
     assert _solves(cubes, sols)
 
 
+def test_autotuner_on_a_small_sample(td4t, seeded):
+    """The autotuner runs its whole grid on a small seeded sample (it raises if
+    any setting changes an optimal length), keeps the fastest trial, and its
+    knobs solve the first C3 cubes to the oracle's lengths."""
+    from cuda_cube_solver_amd import tune
+
+    res = tune.tune(td4t, states.uniform_states(48, 33))
+    assert res["sample"] == 48 and len(res["trials"]) == 7
+    best = res["best"]
+    assert best in res["trials"] and best["cubes_per_s"] == max(t["cubes_per_s"] for t in res["trials"])
+    cubes = states.uniform_states(16, 3)
+    sols = GpuSolver(table=td4t, mode="batched", **tune.knobs(best)).solve(cubes)
+    assert [len(s.moves) for s in sols] == seeded["c3_d4t_seed3_first64"]["lengths"][:16]
+
+
 def test_c1_seed1_on_304_plain_one_at_a_time(seeded, reference_results):
     """C1: 64 seed-1 states on 304, each solved alone in the plain form (one
     launch per IDA* round): the oracle's lengths and completed-round nodes."""
