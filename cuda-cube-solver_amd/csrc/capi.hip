@@ -184,23 +184,17 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
     int rc = tables_setup(t.get(), kind);
     if (rc) return rc;
     const uint64_t n_entries = t->kind.n_entries, n_words = (n_entries + 7) / 8;
+    DeviceBuffers buf;  // build-only memory, freed on every return
     uint32_t* tbl = nullptr;
     unsigned long long* d_cnt = nullptr;
     unsigned int* d_bad = nullptr;
-    auto cleanup = [&]() {
-        hipFree(tbl);
-        hipFree(d_cnt);
-        hipFree(d_bad);
-    };
     auto logf = [&](const std::string& s) {
         if (log) log(s.c_str(), user);
     };
-    if (hipMalloc(&tbl, n_words * 4) != hipSuccess) {
-        cleanup();
+    if (buf.alloc(&tbl, n_words * 4) != hipSuccess)
         return fail(CCS_ENOMEM, "build table allocation (" + std::to_string(n_words * 4) + " bytes)");
-    }
-    HIP_TRY(hipMalloc(&d_cnt, 48 * sizeof(unsigned long long)));
-    HIP_TRY(hipMalloc(&d_bad, sizeof(unsigned int)));
+    HIP_TRY(buf.alloc(&d_cnt, 48 * sizeof(unsigned long long)));
+    HIP_TRY(buf.alloc(&d_bad, sizeof(unsigned int)));
     HIP_TRY(hipMemsetAsync(tbl, 0xFF, n_words * 4, t->stream));
     HIP_TRY(hipMemsetAsync(d_cnt, 0, 48 * sizeof(unsigned long long), t->stream));
     HIP_TRY(hipMemsetAsync(d_bad, 0, sizeof(unsigned int), t->stream));
@@ -215,10 +209,7 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
     uint32_t depth = 0;
     double t_all = now_ms();
     while (remaining) {
-        if (counts.size() >= 16) {
-            cleanup();
-            return fail(CCS_EFAULT, "BFS did not finish within 15 depths");
-        }
+        if (counts.size() >= 16) return fail(CCS_EFAULT, "BFS did not finish within 15 depths");
         // top-down while the frontier is small next to what is left (table.py:112)
         bool fwd = counts.back() < remaining / 4;
         unsigned long long* slot = d_cnt + 1 + depth;
@@ -232,10 +223,7 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
         snprintf(line, sizeof line, "depth %2u %s found %14llu (%7.3fs)", depth + 1, fwd ? "fwd" : "bwd", found,
                  (now_ms() - t0) / 1e3);
         logf(line);
-        if (found == 0) {
-            cleanup();
-            return fail(CCS_EFAULT, std::to_string(remaining) + " entries unreachable");
-        }
+        if (found == 0) return fail(CCS_EFAULT, std::to_string(remaining) + " entries unreachable");
         counts.push_back(found);
         remaining -= found;
         ++depth;
@@ -249,42 +237,47 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
     HIP_TRY(hipStreamSynchronize(t->stream));
     for (int v = 0; v < 16; ++v) {
         uint64_t want = (size_t)v < counts.size() ? counts[v] : 0;
-        if (hist[v] != want) {
-            cleanup();
+        if (hist[v] != want)
             return fail(CCS_EFAULT, "histogram of depth " + std::to_string(v) + " is " + std::to_string(hist[v]) +
                                         ", depth counts say " + std::to_string(want));
-        }
     }
-    // The block table always comes first: the base-3 encoder reads its minima.
+    // Encoders.  The block pass also writes each 124-entry block's minimum to a
+    // byte array, which the base-3 encoder reads for the reference's saturated-
+    // offset values; a base-3-only build runs the pass for the minima alone and
+    // never holds the block table.
+    const uint64_t nb124 = format_blocks(t->kind, CCS_FORMAT_BLOCK);
+    uint8_t* d_min = nullptr;
+    HIP_TRY(buf.alloc(&d_min, nb124));
+    const uint64_t CH = 1ull << 26;  // blocks per launch
     for (int fmt = 0; fmt < 2; ++fmt) {
-        if (fmt == 1 && format == 0) continue;
-        uint64_t nb = format_blocks(t->kind, fmt);
-        if (hipMalloc(&t->d_table[fmt], nb * 32) != hipSuccess) {
-            cleanup();
-            return fail(CCS_ENOMEM, "solving table allocation");
+        const bool keep = format == 2 || format == fmt;
+        if (fmt == 1 && !keep) continue;
+        const uint64_t nb = format_blocks(t->kind, fmt);
+        if (keep) {
+            if (hipMalloc(&t->d_table[fmt], nb * 32) != hipSuccess) {
+                t->d_table[fmt] = nullptr;
+                return fail(CCS_ENOMEM, "solving table allocation");
+            }
+            t->table_bytes[fmt] = nb * 32;
         }
-        t->table_bytes[fmt] = nb * 32;
         const double te = now_ms();
-        const uint64_t CH = 1ull << 26;  // blocks per launch
+        uint32_t* dst = reinterpret_cast<uint32_t*>(t->d_table[fmt]);
         for (uint64_t b0 = 0; b0 < nb; b0 += CH)
-            HIP_TRY(launch_encode(fmt, tbl, reinterpret_cast<const uint32_t*>(t->d_table[0]), n_entries, b0,
-                                  std::min(CH, nb - b0), reinterpret_cast<uint32_t*>(t->d_table[fmt]) + b0 * 8,
+            HIP_TRY(launch_encode(fmt, tbl, d_min, n_entries, b0, std::min(CH, nb - b0), dst ? dst + b0 * 8 : nullptr,
                                   d_bad, t->stream));
         HIP_TRY(hipStreamSynchronize(t->stream));
         char line[160];
-        snprintf(line, sizeof line, "encoded %s table, %llu bytes (%7.3fs)", fmt ? "b3" : "block",
-                 (unsigned long long)(nb * 32), (now_ms() - te) / 1e3);
+        if (keep)
+            snprintf(line, sizeof line, "encoded %s table, %llu bytes (%7.3fs)", fmt ? "b3" : "block",
+                     (unsigned long long)(nb * 32), (now_ms() - te) / 1e3);
+        else
+            snprintf(line, sizeof line, "block minima for the b3 encoder, %llu bytes (%7.3fs)",
+                     (unsigned long long)nb124, (now_ms() - te) / 1e3);
         logf(line);
     }
     unsigned int bad = 0;
     HIP_TRY(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, t->stream));
     HIP_TRY(hipStreamSynchronize(t->stream));
-    cleanup();
-    if (format == 1) {  // only the base-3 table was asked for
-        (void)hipFree(t->d_table[0]);
-        t->d_table[0] = nullptr;
-        t->table_bytes[0] = 0;
-    }
     if (bad) return fail(CCS_EFAULT, "block table has an unset block minimum");
     t->format = format == 2 ? CCS_FORMAT_BLOCK : format;  // both: probes and searches use the block table
     char line[160];
@@ -363,14 +356,15 @@ int ccs_probe(ccs_tables_t* t, const uint8_t* cubes, int n, int move, uint64_t* 
     if (n == 0) return CCS_OK;
     std::vector<Packed> in(n);
     for (int i = 0; i < n; ++i) in[i] = pack(*reinterpret_cast<const Cubie*>(cubes + 40 * (size_t)i));
+    DeviceBuffers buf;
     Frame* d_in = nullptr;
     unsigned long long* d_idx = nullptr;
     uint32_t* d_lb = nullptr;
     Frame* d_state = nullptr;
-    HIP_TRY(hipMalloc(&d_in, sizeof(Frame) * n));
-    HIP_TRY(hipMalloc(&d_idx, 8ull * n));
-    HIP_TRY(hipMalloc(&d_lb, 4ull * n));
-    HIP_TRY(hipMalloc(&d_state, sizeof(Frame) * n));
+    HIP_TRY(buf.alloc(&d_in, sizeof(Frame) * n));
+    HIP_TRY(buf.alloc(&d_idx, 8ull * n));
+    HIP_TRY(buf.alloc(&d_lb, 4ull * n));
+    HIP_TRY(buf.alloc(&d_state, sizeof(Frame) * n));
     HIP_TRY(hipMemcpy(d_in, in.data(), sizeof(Frame) * n, hipMemcpyHostToDevice));
     int fmt = t->format < 0 ? 0 : t->format;
     HIP_TRY(launch_probe(t->K, fmt, t->d_lut, t->d_classidx, t->format < 0 ? nullptr : t->d_table[fmt], d_in, n, move,
@@ -383,10 +377,6 @@ int ccs_probe(ccs_tables_t* t, const uint8_t* cubes, int n, int move, uint64_t* 
         HIP_TRY(hipMemcpy(st.data(), d_state, sizeof(Frame) * n, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) *reinterpret_cast<Cubie*>(state_out + 40 * (size_t)i) = unpack(st[i]);
     }
-    hipFree(d_in);
-    hipFree(d_idx);
-    hipFree(d_lb);
-    hipFree(d_state);
     return CCS_OK;
 }
 
@@ -395,15 +385,14 @@ int ccs_unrank_check(ccs_tables_t* t, const uint64_t* idx, int n, uint64_t* out)
     for (int i = 0; i < n; ++i)
         if (idx[i] >= t->kind.n_entries) return fail(CCS_EINVAL, "index out of range");
     if (n == 0) return CCS_OK;
+    DeviceBuffers buf;
     unsigned long long *d_idx = nullptr, *d_out = nullptr;
-    HIP_TRY(hipMalloc(&d_idx, 8ull * n));
-    HIP_TRY(hipMalloc(&d_out, 8ull * n));
+    HIP_TRY(buf.alloc(&d_idx, 8ull * n));
+    HIP_TRY(buf.alloc(&d_out, 8ull * n));
     HIP_TRY(hipMemcpy(d_idx, idx, 8ull * n, hipMemcpyHostToDevice));
     HIP_TRY(launch_unrank_check(t->K, t->d_lut, t->d_classidx, t->d_class_rep, d_idx, n, d_out, t->stream));
     HIP_TRY(hipStreamSynchronize(t->stream));
     HIP_TRY(hipMemcpy(out, d_out, 8ull * n, hipMemcpyDeviceToHost));
-    hipFree(d_idx);
-    hipFree(d_out);
     return CCS_OK;
 }
 
@@ -412,12 +401,13 @@ int ccs_gather_bench(ccs_tables_t* t, int mode, uint64_t n_probes, int blocks_pe
     if (!t || mode < 0 || mode > 2 || !ms) return fail(CCS_EINVAL, "bad arguments");
     int fmt = t->format < 0 ? 0 : t->format;
     if (!t->d_table[fmt]) return fail(CCS_EINVAL, "no table resident");
+    DeviceBuffers buf;
     unsigned long long* d_sum = nullptr;
-    HIP_TRY(hipMalloc(&d_sum, 8));
+    HIP_TRY(buf.alloc(&d_sum, 8));
     HIP_TRY(hipMemsetAsync(d_sum, 0, 8, t->stream));
     hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(buf.event(&e0));
+    HIP_TRY(buf.event(&e1));
     int grid = device_cus() * (blocks_per_cu > 0 ? blocks_per_cu : 8);
     HIP_TRY(hipEventRecord(e0, t->stream));
     HIP_TRY(launch_gather(mode, grid, reinterpret_cast<const uint32_t*>(t->d_table[fmt]), t->table_bytes[fmt] / 32,
@@ -430,9 +420,6 @@ int ccs_gather_bench(ccs_tables_t* t, int mode, uint64_t n_probes, int blocks_pe
     unsigned long long sum = 0;
     HIP_TRY(hipMemcpy(&sum, d_sum, 8, hipMemcpyDeviceToHost));
     if (checksum) *checksum = sum;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(d_sum);
     return CCS_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/cubesolver.h"
 #include "cube_model.hpp"
@@ -41,6 +42,37 @@ inline int device_cus() {
     }
     return cus;
 }
+
+// Device allocations and events owned by one call: freed on every return path.
+struct DeviceBuffers {
+    std::vector<void*> ptrs;
+    std::vector<hipEvent_t> events;
+    template <typename T>
+    hipError_t alloc(T** p, size_t bytes) {
+        *p = nullptr;
+        hipError_t e = hipMalloc((void**)p, bytes);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+    hipError_t event(hipEvent_t* ev) {
+        hipError_t e = hipEventCreate(ev);
+        if (e == hipSuccess) events.push_back(*ev);
+        return e;
+    }
+    // Stop owning p (its ownership moves elsewhere).
+    void release(void* p) {
+        for (auto& q : ptrs)
+            if (q == p) q = nullptr;
+    }
+    DeviceBuffers() = default;
+    DeviceBuffers(const DeviceBuffers&) = delete;
+    DeviceBuffers& operator=(const DeviceBuffers&) = delete;
+    ~DeviceBuffers() {
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
 
 inline int kind_K(const Kind& k) { return k.is304 ? 0 : k.nec; }
 

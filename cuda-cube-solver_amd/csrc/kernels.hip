@@ -387,7 +387,8 @@ __device__ __forceinline__ void stage_span(const uint32_t* __restrict__ tbl, uns
 
 // Block format, one block of fewer than 124 entries (the table's last).
 __device__ void encode_block_partial(const uint32_t* __restrict__ tbl, unsigned long long first, uint32_t n,
-                                     uint32_t* __restrict__ out, unsigned int* __restrict__ bad) {
+                                     uint32_t* __restrict__ out, uint8_t* __restrict__ mn_out,
+                                     unsigned int* __restrict__ bad) {
     uint32_t m = 15;
     for (uint32_t j = 0; j < n; ++j) m = min(m, nib_get(tbl, first + j));
     uint32_t w[8] = {m, 0, 0, 0, 0, 0, 0, 0};
@@ -399,16 +400,20 @@ __device__ void encode_block_partial(const uint32_t* __restrict__ tbl, unsigned 
         w[pos >> 5] |= min(v - m, 3u) << (pos & 31u);
     }
     if (unset) atomicOr(bad, 1u);
+    *mn_out = (uint8_t)m;
+    if (!out) return;
 #pragma unroll
     for (int k = 0; k < 8; ++k) out[k] = w[k];
 }
 
 // Block format (kernels.cu:396-406, 485-503 define it): per 124 entries, u8
 // minimum then 2-bit offsets min(v - min, 3) at bit 8 + 2j.  Blocks b0 .. b0 + nb - 1.
+// Each block's minimum also goes to mins[block] (the base-3 encoder reads them);
+// with out == nullptr only the minima are written.
 __global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_words,
                                                            unsigned long long n_entries, unsigned long long b0,
                                                            unsigned long long nb, uint32_t* __restrict__ out,
-                                                           unsigned int* __restrict__ bad) {
+                                                           uint8_t* __restrict__ mins, unsigned int* __restrict__ bad) {
     __shared__ uint32_t stage[4][1024];  // 64 blocks x 15.5 words, plus the word after
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const unsigned long long i0 = ((unsigned long long)blockIdx.x * 4u + wave) * 64u;  // wave's first block - b0
@@ -418,9 +423,9 @@ __global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __res
     const unsigned long long i = i0 + lane;
     if (i >= nb) return;
     const unsigned long long first = (gb0 + lane) * BLK_ENTRIES;
-    uint32_t* o = out + i * 8u;
+    uint32_t* o = out ? out + i * 8u : nullptr;
     if (n_entries - first < BLK_ENTRIES) {
-        encode_block_partial(tbl, first, (uint32_t)(n_entries - first), o, bad);
+        encode_block_partial(tbl, first, (uint32_t)(n_entries - first), o, mins + gb0 + lane, bad);
         return;
     }
     const uint32_t* st = stage[wave] + ((lane * 31u) >> 1);  // block's first word within the span
@@ -429,6 +434,8 @@ __global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __res
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = funnel(st[k + 1], st[k], sh);
     if (encode_block_words(a, w)) atomicOr(bad, 1u);
+    mins[gb0 + lane] = (uint8_t)w[0];
+    if (!o) return;
     reinterpret_cast<uint4*>(o)[0] = make_uint4(w[0], w[1], w[2], w[3]);
     reinterpret_cast<uint4*>(o)[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
@@ -437,7 +444,7 @@ __global__ void __launch_bounds__(256) encode_block_kernel(const uint32_t* __res
 // m | c << 6 with the cut c in 0..3 maximising the sum of encoded bounds
 // (lowest c on ties), then 31 bytes of five digits, little-endian base 3.
 __global__ void __launch_bounds__(256) encode_b3_kernel(const uint32_t* __restrict__ tbl, unsigned long long n_words,
-                                                        const uint32_t* __restrict__ blk, unsigned long long n_entries,
+                                                        const uint8_t* __restrict__ mins, unsigned long long n_entries,
                                                         unsigned long long b0, unsigned long long nb,
                                                         uint32_t* __restrict__ out) {
     __shared__ uint32_t stage[4][1280];  // 64 blocks x 19.375 words, plus the word after
@@ -451,12 +458,12 @@ __global__ void __launch_bounds__(256) encode_b3_kernel(const uint32_t* __restri
     const unsigned long long first = (gb0 + lane) * B3_ENTRIES;
     const uint32_t n = (uint32_t)min((unsigned long long)B3_ENTRIES, n_entries - first);
     uint32_t* o = out + i * 8u;
-    // the block's entries meet at most three 124-entry blocks of the block table
+    // the block's entries meet at most three 124-entry blocks of the block format
     const unsigned long long bl = first / BLK_ENTRIES;
     const uint32_t j1 = (uint32_t)((bl + 1u) * BLK_ENTRIES - first), j2 = j1 + BLK_ENTRIES;
-    const uint32_t mn0 = blk[bl * 8u] & 0xFFu;
-    const uint32_t mn1 = j1 < n ? blk[(bl + 1u) * 8u] & 0xFFu : 0u;
-    const uint32_t mn2 = j2 < n ? blk[(bl + 2u) * 8u] & 0xFFu : 0u;
+    const uint32_t mn0 = mins[bl];
+    const uint32_t mn1 = j1 < n ? mins[bl + 1u] : 0u;
+    const uint32_t mn2 = j2 < n ? mins[bl + 2u] : 0u;
     auto put = [&](uint32_t k, uint32_t word) __attribute__((always_inline)) { o[k] = word; };
     if (n == B3_ENTRIES) {
         const uint32_t off = lane * B3_ENTRIES;  // block's first nibble within the span
@@ -1674,15 +1681,16 @@ hipError_t launch_gather(int mode, int grid, const uint32_t* blk, unsigned long 
     return hipGetLastError();
 }
 
-hipError_t launch_encode(int format, const uint32_t* tbl, const uint32_t* blk, unsigned long long n_entries,
+hipError_t launch_encode(int format, const uint32_t* tbl, uint8_t* mins, unsigned long long n_entries,
                          unsigned long long b0, unsigned long long nb, uint32_t* out, unsigned int* bad, hipStream_t s) {
     if (nb == 0) return hipSuccess;
     if (b0 % 64u) return hipErrorInvalidValue;  // a wave's span must start on a 16-byte boundary
     const unsigned long long n_words = (n_entries + 7u) / 8u;
     dim3 grid((unsigned)((nb + 255) / 256)), block(256);
     if (format == 0)
-        hipLaunchKernelGGL(encode_block_kernel, grid, block, 0, s, tbl, n_words, n_entries, b0, nb, out, bad);
-    else hipLaunchKernelGGL(encode_b3_kernel, grid, block, 0, s, tbl, n_words, blk, n_entries, b0, nb, out);
+        hipLaunchKernelGGL(encode_block_kernel, grid, block, 0, s, tbl, n_words, n_entries, b0, nb, out, mins, bad);
+    else if (out) hipLaunchKernelGGL(encode_b3_kernel, grid, block, 0, s, tbl, n_words, mins, n_entries, b0, nb, out);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -34,9 +34,10 @@ hipError_t launch_bfs_layer(int K, bool forward, int grid, const CubeLUT* lut, u
                             unsigned long long n_entries, uint32_t depth, unsigned long long* counter, hipStream_t s);
 hipError_t launch_histogram(int grid, const uint32_t* tbl, unsigned long long n_words, unsigned long long n_entries,
                             unsigned long long* counts, hipStream_t s);
-// format 0 writes the block table; format 1 the base-3 table, which reads the
-// block table (blk) for the reference's saturated-offset values.
-hipError_t launch_encode(int format, const uint32_t* tbl, const uint32_t* blk, unsigned long long n_entries,
+// format 0 writes the block table (out; none if null) and every block's
+// minimum (mins, one byte per 124-entry block); format 1 writes the base-3
+// table, reading those minima for the reference's saturated-offset values.
+hipError_t launch_encode(int format, const uint32_t* tbl, uint8_t* mins, unsigned long long n_entries,
                          unsigned long long b0, unsigned long long nb, uint32_t* out, unsigned int* bad, hipStream_t s);
 
 // mode 0: random probes; 1: random probes with nontemporal loads; 2: stream the table once.

@@ -25,19 +25,6 @@ namespace {
 
 __global__ void read_clock_kernel(unsigned long long* out) { *out = wall_clock64(); }
 
-struct DeviceBuffers {
-    std::vector<void*> ptrs;
-    template <typename T>
-    hipError_t alloc(T** p, size_t bytes) {
-        hipError_t e = hipMalloc((void**)p, bytes);
-        if (e == hipSuccess) ptrs.push_back(*p);
-        return e;
-    }
-    ~DeviceBuffers() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-};
-
 void fill_opts(const ccs_solve_opts_t* in, ccs_solve_opts_t& o) {
     memset(&o, 0, sizeof o);
     o.mode = 0;
@@ -146,8 +133,8 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     HIP_TRY(buf.alloc(&d_stats, 4 * sizeof(unsigned long long)));
     HIP_TRY(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), t->stream));
     hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(buf.event(&ev0));
+    HIP_TRY(buf.event(&ev1));
     double kernel_ms = 0;
     uint32_t launches = 0;
 
@@ -250,12 +237,16 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
         HIP_TRY(hipStreamSynchronize(t->stream));
     } else {
         // ---- persistent batched: all rounds on the device ----
+        // A cube whose root bound is already above the limit is given up without
+        // a round, as the plain form does (its first round would search at that
+        // bound and could return a solution longer than the limit).
         std::vector<int> live;
         for (int i = 0; i < n; ++i) {
             if (solved0[i]) {
                 done_ms[i] = 0;
                 continue;
             }
+            if ((int)jobs[i].bound > A.max_bound) continue;
             live.push_back(i);
         }
         const int nl = (int)live.size();
@@ -364,9 +355,6 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
             }
         }
     }
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-
     // Verify every solution (solver.py:317-319).
     int bad = -1;
     for (int i = 0; i < n; ++i) {

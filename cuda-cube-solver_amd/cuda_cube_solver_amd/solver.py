@@ -9,7 +9,6 @@ cube's IDA* rounds on the device.
 from __future__ import annotations
 
 import ctypes
-import re
 import time
 from dataclasses import dataclass, field
 from pathlib import Path
@@ -37,34 +36,45 @@ class NoTableFits(RuntimeError):
     """No table preset fits in the device memory budget."""
 
 
-def _table_file(tables: Path, name: str) -> Path:
-    path = tables / name
-    if not path.exists():
-        m = re.fullmatch(r"prune-ud-(\d+|[a-z0-9]+)(?:-blk)?(-b3)?\.bin", name)
-        hint = f"; build it with `cuda-cube-solver build-table --table {m[1]}{m[2] or ''} --out {tables}`" if m else ""
-        raise TableNotFound(f"{path} not found{hint}")
-    return path
+def _table_path(directory: Path, filename: str) -> Path:
+    """``directory / filename``, or TableNotFound with the command that builds
+    it (the message of the reference's solver.py:82-88)."""
+    path = directory / filename
+    if path.exists():
+        return path
+    hint = ""
+    stem = filename[len("prune-ud-"):-len(".bin")] if filename.startswith("prune-ud-") and filename.endswith(".bin") else ""
+    for suffix, keep in (("-blk", ""), ("-b3", "-b3")):
+        if stem.endswith(suffix) and stem[: -len(suffix)]:
+            hint = (f"; build it with `cuda-cube-solver build-table --table {stem[: -len(suffix)]}{keep} "
+                    f"--out {directory}`")
+            break
+    raise TableNotFound(f"{path} not found{hint}")
 
 
 def open_solver(tables: Path = Path("tables"), memory="auto", table: str | None = None, build: bool = True,
                 log=print) -> "GpuSolver":
-    """A solver with the fastest preset that fits ``memory`` (GB or "auto"), or the preset ``table``.
-    A missing table is built first unless ``build`` is False."""
-    from . import table as tables_mod
+    """A solver on the preset ``table``, or on the fastest one that fits
+    ``memory`` (GB, or "auto" for the free device memory).  A missing table
+    file is built first, unless ``build`` is False."""
+    from . import table as table_io
 
-    tables = Path(tables)
-    if tables.suffix == ".bin":
-        tables = tables.parent
-    preset = kinds.pick(tables, memory, table)[0]
+    directory = Path(tables)
+    if directory.suffix == ".bin":  # a table file was named: use its directory
+        directory = directory.parent
+    preset, _ = kinds.pick(directory, memory, table)
     if preset is None:
         raise NoTableFits("no table fits in that GPU memory (the smallest, 304, needs 1.8 GB)")
-    if not (tables / preset.file).exists():
+    target = directory / preset.file
+    if not target.exists():
         if not build:
-            raise TableNotFound(f"{tables / preset.file} not found")
-        log(f"building table {preset.name} ({preset.table_bytes / 1e9:.2f} GB, needs "
-            f"{preset.build_bytes / 1e9:.1f} GB of device memory) in {tables}/")
-        log(f"wrote {tables_mod.build_preset(tables, preset, log=lambda m: log(f'  {m}'))}")
-    return GpuSolver(tables, kind=preset.kind, table_format=preset.format)
+            raise TableNotFound(f"{target} not found")
+        gb = 1e9
+        log(f"building table {preset.name} ({preset.table_bytes / gb:.2f} GB, needs "
+            f"{preset.build_bytes / gb:.1f} GB of device memory) in {directory}/")
+        written = table_io.build_preset(directory, preset, log=lambda line: log("  " + line))
+        log(f"wrote {written}")
+    return GpuSolver(directory, kind=preset.kind, table_format=preset.format)
 
 
 class Tables:
@@ -156,7 +166,7 @@ class GpuSolver:
             if tables.suffix == ".bin":
                 tables = tables.parent
             name = f"prune-ud-{kind}-{'b3' if table_format == 'b3' else 'blk'}.bin"
-            table = Tables.open(_table_file(tables, name), kind, table_format)
+            table = Tables.open(_table_path(tables, name), kind, table_format)
         self.table = table
         self.kind = table.kind
         self.mode = mode

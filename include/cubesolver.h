@@ -27,8 +27,8 @@ extern "C" {
 #endif
 
 #define CCS_OK 0
-#define CCS_ENOENT (-2)   /* table file missing  (solver.py:78-88 TableNotFound) */
-#define CCS_EIO (-5)      /* file size or read error (table.py:353-363 ValueError) */
+#define CCS_ENOENT (-2)   /* table file missing  (solver.py:82-88 TableNotFound) */
+#define CCS_EIO (-5)      /* file size or read error (table.py:36-37 ValueError) */
 #define CCS_ENOMEM (-12)  /* device memory budget / allocation (kinds.py:184-195, NoTableFits) */
 #define CCS_EFAULT (-14)  /* self-check failed (table.py:183-189, solver.py:317-319 AssertionError) */
 #define CCS_EINVAL (-22)  /* bad name or argument (kinds.py:203-206 ValueError) */

@@ -207,6 +207,25 @@ def test_max_bound_below_the_optimum_is_a_clean_error(td4t, reference_results):
     assert [len(s.moves) for s in sols] == [17] + [r["len"] for r in ref["results"][300:303]]
 
 
+def test_root_bound_above_max_bound_is_refused_in_both_forms(td4t):
+    """A cube whose root bound already exceeds max_bound is given up without a
+    round in both forms (a first round at the root bound would otherwise return
+    a solution longer than the limit)."""
+    rng = states.SplitMix64(71)
+    short = [apply_moves(CubieState(), [MOVES_HTM[rng.below(18)] for _ in range(5)]) for _ in range(24)]
+    sols = GpuSolver(table=td4t, mode="plain").solve(short)
+    # cubes solved in their first round: root bound == optimal length
+    tight = [(c, len(s.moves)) for c, s in zip(short, sols) if s.iteration_nodes and
+             s.iteration_nodes[0][0] == len(s.moves) and len(s.moves) >= 2]
+    assert tight, "no short scramble solved in its first round"
+    cube, length = tight[0]
+    for mode in ("plain", "batched"):
+        with pytest.raises(AssertionError, match="not solved within max_bound"):
+            GpuSolver(table=td4t, mode=mode, max_bound=length - 1).solve([cube])
+        got = GpuSolver(table=td4t, mode=mode, max_bound=length).solve([cube])
+        assert len(got[0].moves) == length
+
+
 def test_server_stops_promptly_while_busy(td4t):
     """Stopping a server with hard cubes still in flight returns within
     seconds (pending and unfinished cubes are dropped); a new server on the

@@ -114,17 +114,51 @@ def test_presets_match_reference(index_vectors):
 
 
 def test_preset_choice(tmp_path):
+    """The reference's ladder and rules (kinds.py:184-211), with this library's
+    build memory: the exact-depth build table takes 4 bits per entry, so a
+    16 GB card holds d4t but cannot build it here (the reference's 2-bit build
+    can: tests/test_cuda_cube_solver.py:169-182 expects d4t).  On an MI355X
+    every preset builds."""
     names = lambda pair: tuple(p.name if p else None for p in pair)
-    # MI355X-sized budgets: everything builds
     assert names(kinds.choose(tmp_path, int(200e9))) == ("d4t", None)
-    # a 16 GB budget holds d4t but cannot build it (exact-depth build table, 4 bits/entry)
     assert names(kinds.choose(tmp_path, int(16e9))) == ("a4t", "d4t")
+    assert names(kinds.choose(tmp_path, int(8e9), int(16e9))) == ("a4t", "d4t-b3")
     (tmp_path / "prune-ud-d4t-blk.bin").touch()
     assert names(kinds.choose(tmp_path, int(16e9))) == ("d4t", None)
     assert names(kinds.choose(tmp_path, int(16e9), to_build=True)) == ("a4t", "d4t")
+    (tmp_path / "prune-ud-d4t-b3.bin").touch()
+    assert names(kinds.choose(tmp_path, int(8e9))) == ("d4t-b3", None)
     assert names(kinds.choose(tmp_path, int(1e9))) == (None, None)
     with pytest.raises(ValueError, match="unknown table"):
         kinds.pick(tmp_path, name="nope")
+
+
+def test_b3_build_memory_at_the_budget_edge(tmp_path):
+    """A base-3 build holds the nibble build table, the base-3 table and one
+    minimum byte per 124-entry block (no block table): a budget one byte short
+    of that falls back to the next preset down."""
+    s4 = kinds.PRESET_BY_NAME["s4-b3"]
+    n = s4.entries
+    assert s4.build_bytes == 4 * ((n + 7) // 8) + 32 * ((n + 154) // 155) + (n + 123) // 124
+    need = s4.build_bytes + kinds.SOLVE_MARGIN
+    names = lambda pair: tuple(p.name if p else None for p in pair)
+    assert names(kinds.choose(tmp_path, need))[0] == "s4-b3"
+    assert names(kinds.choose(tmp_path, need - 1))[0] == "304"
+    # the block preset of a kind needs its block table instead
+    d4t = kinds.PRESET_BY_NAME["d4t"]
+    assert d4t.build_bytes == 4 * ((d4t.entries + 7) // 8) + d4t.table_bytes
+
+
+def test_cli_tables_listing(tmp_path):
+    env = {"PYTHONPATH": str(ROOT / "cuda-cube-solver_amd")}
+    out = subprocess.run([sys.executable, "-c",
+                          "from cuda_cube_solver_amd import kinds; print('\\n'.join(kinds.describe(%r)))" % str(tmp_path)],
+                         capture_output=True, text=True, env=env)
+    lines = out.stdout.splitlines()
+    assert lines[0].split() == ["preset", "VRAM", "cubes/s", "superflip", "build", "VRAM", "file"]
+    assert lines[1].startswith("304        1.42GB      6.0      44.6s")
+    assert lines[1].endswith("prune-ud-304-blk.bin  (not built)")
+    assert lines[-1].endswith("prune-ud-d4-blk.bin  (not built)  (only with --table)")
 
 
 def test_facelets_match_reference(index_vectors):
