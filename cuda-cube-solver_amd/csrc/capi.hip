@@ -178,6 +178,8 @@ int ccs_tables_open(const char* path, const char* kind, int format, ccs_tables_t
 int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, uint64_t* depth_counts,
                      int* n_depths, ccs_tables_t** out) {
     if (!kind || !out) return fail(CCS_EINVAL, "null argument");
+    const bool keep_depths = (format & CCS_BUILD_KEEP_DEPTHS) != 0;
+    format &= ~CCS_BUILD_KEEP_DEPTHS;
     if (format < 0 || format > 2) return fail(CCS_EINVAL, "unknown table format");
     *out = nullptr;
     std::unique_ptr<ccs_tables> t(new ccs_tables());
@@ -280,6 +282,11 @@ int ccs_tables_build(const char* kind, int format, ccs_log_fn log, void* user, u
     HIP_TRY(hipStreamSynchronize(t->stream));
     if (bad) return fail(CCS_EFAULT, "block table has an unset block minimum");
     t->format = format == 2 ? CCS_FORMAT_BLOCK : format;  // both: probes and searches use the block table
+    if (keep_depths) {  // the handle takes the build table
+        buf.release(tbl);
+        t->d_depths = tbl;
+        t->depth_bytes = n_words * 4;
+    }
     char line[160];
     snprintf(line, sizeof line, "built %s in %.1fs", kind, (now_ms() - t_all) / 1e3);
     logf(line);
@@ -302,14 +309,26 @@ int ccs_tables_use(ccs_tables_t* t, int format) {
 }
 
 uint64_t ccs_tables_bytes(ccs_tables_t* t, int format) {
-    if (!t || format < 0 || format > 1) return 0;
+    if (!t) return 0;
+    if (format == CCS_TABLE_DEPTHS) return t->depth_bytes;
+    if (format < 0 || format > 1) return 0;
     return t->table_bytes[format];
 }
 
 int ccs_tables_read(ccs_tables_t* t, int format, uint64_t offset, uint64_t n, void* host) {
-    if (!t || format < 0 || format > 1 || !t->d_table[format]) return fail(CCS_EINVAL, "table not resident");
-    if (offset + n > t->table_bytes[format]) return fail(CCS_EINVAL, "read past the end of the table");
-    HIP_TRY(hipMemcpy(host, t->d_table[format] + offset, n, hipMemcpyDeviceToHost));
+    if (!t || (n && !host)) return fail(CCS_EINVAL, "bad arguments");
+    const uint8_t* src = nullptr;
+    uint64_t size = 0;
+    if (format == CCS_TABLE_DEPTHS) {
+        src = reinterpret_cast<const uint8_t*>(t->d_depths);
+        size = t->depth_bytes;
+    } else if (format == 0 || format == 1) {
+        src = t->d_table[format];
+        size = t->table_bytes[format];
+    }
+    if (!src) return fail(CCS_EINVAL, "table not resident");
+    if (offset > size || n > size - offset) return fail(CCS_EINVAL, "read past the end of the table");
+    HIP_TRY(hipMemcpy(host, src + offset, n, hipMemcpyDeviceToHost));
     return CCS_OK;
 }
 

@@ -96,6 +96,8 @@ struct ccs_tables {
     ccs::ClassLUT* d_class_lut = nullptr;  // corner classes from small tables, for the search's LDS
     uint8_t* d_table[2] = {nullptr, nullptr};
     uint64_t table_bytes[2] = {0, 0};
+    uint32_t* d_depths = nullptr;  // the BFS build table, kept only on request (CCS_BUILD_KEEP_DEPTHS)
+    uint64_t depth_bytes = 0;
     unsigned long long* d_prefix[ccs::MAX_PLEN + 1] = {};  // packed prefixes, 8 B per item
     uint32_t prefix_n[ccs::MAX_PLEN + 1] = {};
     hipStream_t stream = nullptr;
@@ -108,6 +110,7 @@ struct ccs_tables {
         hipFree(d_class_stab);
         hipFree(d_class_lut);
         for (auto* p : d_table) hipFree(p);
+        hipFree(d_depths);
         for (auto* p : d_prefix) hipFree(p);
         if (stream) hipStreamDestroy(stream);
     }

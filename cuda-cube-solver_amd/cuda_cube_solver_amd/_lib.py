@@ -23,6 +23,8 @@ CCS_EINVAL = -22
 CCS_EDEVICE = -100
 FORMAT_BLOCK = 0
 FORMAT_B3 = 1
+TABLE_DEPTHS = 2
+BUILD_KEEP_DEPTHS = 4
 
 
 class LibraryMissing(RuntimeError):

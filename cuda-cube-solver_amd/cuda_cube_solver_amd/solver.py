@@ -98,13 +98,14 @@ class Tables:
         return cls(h.value, kind, fmt)
 
     @classmethod
-    def build(cls, kind: str, fmt: str, log=None) -> tuple["Tables", list[int]]:
-        """BFS on the device; fmt "block", "b3" or "both" (both kept resident)."""
+    def build(cls, kind: str, fmt: str, log=None, keep_depths: bool = False) -> tuple["Tables", list[int]]:
+        """BFS on the device; fmt "block", "b3" or "both" (both kept resident).
+        ``keep_depths`` also keeps the exact-depth build table (read("depths", ...))."""
         h = ctypes.c_void_p()
         counts = (ctypes.c_uint64 * 32)()
         nd = ctypes.c_int(32)
         cb = _lib.LOG_FN(lambda line, user: log(line.decode()) if log else None)
-        code = {"block": 0, "b3": 1, "both": 2}[fmt]
+        code = {"block": 0, "b3": 1, "both": 2}[fmt] | (_lib.BUILD_KEEP_DEPTHS if keep_depths else 0)
         rc = _lib.lib().ccs_tables_build(kind.encode(), code, cb, None, counts, ctypes.byref(nd), ctypes.byref(h))
         if rc == _lib.CCS_EFAULT:
             raise AssertionError(_lib.lib().ccs_last_error().decode())
@@ -122,12 +123,17 @@ class Tables:
         _lib.check(_lib.lib().ccs_tables_use(self.handle, 1 if fmt == "b3" else 0))
         self.format = fmt
 
+    @staticmethod
+    def _code(fmt: str) -> int:
+        return {"block": 0, "b3": 1, "depths": _lib.TABLE_DEPTHS}[fmt]
+
     def nbytes(self, fmt: str | None = None) -> int:
-        return int(_lib.lib().ccs_tables_bytes(self.handle, 1 if (fmt or self.format) == "b3" else 0))
+        return int(_lib.lib().ccs_tables_bytes(self.handle, self._code(fmt or self.format)))
 
     def read(self, fmt: str, offset: int, n: int) -> bytes:
+        """n bytes at offset of the "block" or "b3" table, or of the kept "depths" build table."""
         buf = ctypes.create_string_buffer(n)
-        _lib.check(_lib.lib().ccs_tables_read(self.handle, 1 if fmt == "b3" else 0, offset, n, buf))
+        _lib.check(_lib.lib().ccs_tables_read(self.handle, self._code(fmt), offset, n, buf))
         return buf.raw
 
     def save(self, fmt: str, path: Path) -> None:

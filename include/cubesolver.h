@@ -37,6 +37,10 @@ extern "C" {
 
 #define CCS_FORMAT_BLOCK 0 /* prune-ud-<kind>-blk.bin: u8 minimum + 124 two-bit offsets per 32 B */
 #define CCS_FORMAT_B3 1    /* prune-ud-<kind>-b3.bin: header + 155 base-3 digits per 32 B */
+#define CCS_TABLE_DEPTHS 2 /* ccs_tables_read/bytes: the BFS build table, an exact depth per entry in
+                              a nibble (8 entries per u32, entry j at bits 4 (j mod 8)); resident only
+                              if the build was asked to keep it */
+#define CCS_BUILD_KEEP_DEPTHS 4 /* OR'ed into ccs_tables_build's format: keep the build table resident */
 
 typedef struct ccs_tables ccs_tables_t;
 typedef struct ccs_server ccs_server_t;
@@ -100,10 +104,13 @@ int ccs_tables_save(ccs_tables_t* t, int format, const char* path);
  * Replaces picking the table file by format in GpuSolver (solver.py:164-169). */
 int ccs_tables_use(ccs_tables_t* t, int format);
 
-/* Bytes of a resident solving table (0 if that format is not resident). */
+/* Bytes of a resident solving table, or of the kept build table with
+ * CCS_TABLE_DEPTHS (0 if not resident). */
 uint64_t ccs_tables_bytes(ccs_tables_t* t, int format);
 
-/* Copy n bytes at offset of a resident solving table to host memory. */
+/* Copy n bytes at offset of a resident solving table (or, with
+ * CCS_TABLE_DEPTHS, of the kept build table) to host memory.  The reference
+ * reads its tables back with CuPy .get() (table.py:20-25). */
 int ccs_tables_read(ccs_tables_t* t, int format, uint64_t offset, uint64_t n, void* host);
 
 void ccs_tables_close(ccs_tables_t* t);

@@ -82,10 +82,20 @@ def _probe(tb, sts, move):
     return idx, lb, [CubieState.from_bytes(out.raw[40 * i:40 * i + 40]) for i in range(n)]
 
 
+@pytest.fixture(scope="module")
+def b3_304(t304):
+    return np.frombuffer(t304.read("b3", 0, t304.nbytes("b3")), dtype=np.uint8)
+
+
+@pytest.mark.parametrize("fmt", ["block", "b3"])
 @pytest.mark.parametrize("move", [-1] + list(range(18)) + [-2 - m for m in range(18)])
-def test_probe_moves_index_and_bound_match_oracle(t304, blk304, move):
+def test_probe_moves_index_and_bound_match_oracle(t304, blk304, b3_304, move, fmt):
     sts = states.uniform_states(300, 21) + states.flip_heavy_states(20, 22)
-    idx, lb, moved = _probe(t304, sts, move)
+    t304.use(fmt)
+    try:
+        idx, lb, moved = _probe(t304, sts, move)
+    finally:
+        t304.use("block")
     for s, got in zip(sts, moved):
         if move >= 0:
             want = s.then(MOVES_HTM[move])
@@ -97,7 +107,8 @@ def test_probe_moves_index_and_bound_match_oracle(t304, blk304, move):
     k = O.kind("304")
     want_idx = O.table_index(k, _arr(moved))
     assert (idx == want_idx).all()
-    assert [int(v) for v in lb] == [O.lookup(blk304, 0, int(i)) for i in want_idx]
+    table, code = (blk304, 0) if fmt == "block" else (b3_304, 1)
+    assert [int(v) for v in lb] == [O.lookup(table, code, int(i)) for i in want_idx]
 
 
 def test_unrank_inverts_the_index_on_device(t304, td4t):
