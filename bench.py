@@ -6,11 +6,15 @@ states, seed 3, on d4t, one persistent launch).  Before it: the d4t tables
 `table_build`, and the autotuner picks the persistent form's knobs on a
 separate 2,048-state sample (seed 33: large enough that its final rounds are
 ordered by item keys as the workload's are); neither is part of the step.
-After it:
-the first 128 states solved one at a time in the plain form, the random-probe
-ceiling of the memory system, a Poisson serving run at 50% and 80% of the
-measured capacity (seed 5), and the CPU oracle on a bounded sample.  Prints
-one JSON line.
+After it: every workload state solved one at a time in the plain form (all
+lengths compared with the batched ones), the random-probe ceiling of the
+memory system, the hard tail (256 length >= 18 positions plus 64 flip-heavy
+states on d4, the ten distance-20 positions and the superflip each alone),
+the cold and warm load time of the d4t file, and a Poisson serving run at 50%
+and 80% of the measured capacity (seed 5).  The CPU baseline (the oracle's
+complete solves of the first 64 workload states on the host cores) runs in a
+thread beside the batched steps, whose time is device time.  Prints one JSON
+line.
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1]
 """
@@ -103,25 +107,165 @@ def gather_ceiling(tb) -> dict:
     return out
 
 
-def cpu_baseline(tb, cubes, nodes_per_cube: float, seconds_budget: float):
-    """The oracle (C restatement of the reference's search) on the host cores,
-    over the same GPU-built table: one workload cube per thread for a fixed
-    wall-clock budget.  Its node rate divided by the workload's nodes per
-    cube (the same node definition on both sides) gives cubes/s."""
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle as O
 
-    table = np.frombuffer(tb.read("block", 0, tb.nbytes("block")), dtype=np.uint8)
-    k = O.kind("d4t")
-    threads = min(os.cpu_count() or 1, 16)
-    arr = O.cubes([(s.cp, s.co, s.ep, s.eo) for s in cubes[:threads]])
-    nodes, dt, _ = O.timed_nodes(k, table, 0, arr, seconds_budget, threads)
-    rate = nodes / dt
-    return {"value": round(rate / nodes_per_cube, 4), "unit": "cubes/s", "cores": threads, "kind": "port",
-            "sample": f"first {threads} workload states, one per thread, {dt:.1f} s wall: {nodes / 1e6:.0f} M nodes "
-                      f"= {rate / 1e6:.2f} M nodes/s, divided by the workload's {nodes_per_cube / 1e6:.1f} M nodes/cube"}
+def host_threads() -> int:
+    """Cores this process may use: the affinity mask, capped by OMP_NUM_THREADS
+    when it is set (the share of the host that a one-GPU job gets)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
+
+
+class CpuBaseline:
+    """The oracle (C restatement of the reference's step-mode search) solving
+    the first n workload states completely on the host cores, one cube per
+    thread, over the same GPU-built table.  Runs in a thread (the oracle's C
+    call releases the GIL) while the GPU measures device time; two cores stay
+    free for the GPU host thread."""
+
+    def __init__(self, tb, cubes, n_threads: int):
+        import threading
+
+        import numpy as np
+
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        self.n_threads = n_threads
+        self.table = np.frombuffer(tb.read("block", 0, tb.nbytes("block")), dtype=np.uint8)
+        self.arr = O.cubes([(s.cp, s.co, s.ep, s.eo) for s in cubes])
+        self.kind = O.kind("d4t")
+        self.result = None
+        self.error = None
+        self.thread = threading.Thread(target=self._run, daemon=True)
+        self.thread.start()
+
+    def _run(self):
+        import oracle as O
+
+        try:
+            t = time.perf_counter()
+            res, bad = O.solve_many(self.kind, self.table, 0, self.arr, n_threads=self.n_threads)
+            wall = time.perf_counter() - t
+            self.result = (res, bad, wall)
+        except Exception as e:  # noqa: BLE001
+            self.error = e
+
+    def finish(self, gpu_lengths) -> dict:
+        self.thread.join()
+        if self.error is not None:
+            raise self.error
+        res, bad, wall = self.result
+        secs = [r["seconds"] * 1e3 for r in res]
+        nodes = sum(r["nodes"] for r in res)
+        n = len(res)
+        return {"value": round(n / wall, 4), "unit": "cubes/s", "cores": self.n_threads, "kind": "port",
+                "cpu": cpu_model(),
+                "sample": f"first {n} workload states (seed 3) on d4t, complete solves, one cube per thread "
+                          f"on {self.n_threads} threads: {wall:.1f} s wall, {nodes / 1e9:.2f} G nodes "
+                          f"({nodes / wall / 1e6:.1f} M nodes/s)",
+                "p50_ms": round(pct(secs, 50), 1), "p99_ms": round(pct(secs, 99), 1),
+                "unsolved": bad,
+                "lengths_equal_gpu": [r["length"] for r in res] == list(gpu_lengths[:n])}
+
+
+def hard_tail(tb_d4t, log) -> dict:
+    """Hard positions: C4 on d4 (256 of the reference's length >= 18 positions
+    of data/random1000.txt drawn with seed 4, plus 64 flip-heavy states, one
+    persistent launch), the ten distance-20 positions and the superflip each
+    solved alone in the persistent form on d4t and d4.  Lengths are checked
+    against the reference's recorded ones."""
+    from cuda_cube_solver_amd import states
+    from cuda_cube_solver_amd.cubie import MOVES_HTM, CubieState, apply_moves, parse_moves
+    from cuda_cube_solver_amd.solver import GpuSolver, Tables
+
+    golden = json.loads((ROOT / "tests" / "golden" / "reference_results.json").read_text())
+
+    def scrambled(texts):
+        return [apply_moves(CubieState(), [MOVES_HTM[m] for m in parse_moves(t)]) for t in texts]
+
+    ref = golden["random1000"]
+    pool = [i for i, r in enumerate(ref["results"]) if r["len"] >= 18]
+    pick = [pool[i] for i in states.sample_indices(len(pool), 256, 4)]
+    c4 = scrambled([ref["scrambles"][i] for i in pick]) + states.flip_heavy_states(64, 4)
+    d20 = scrambled(golden["distance20_random10"]["scrambles"])
+    sf = states.flip_heavy_states(2, 1)[-1]
+    out = {}
+    td4, _ = Tables.build("d4", "block")
+    try:
+        for name, tb in (("d4", td4), ("d4t", tb_d4t)):
+            solver = GpuSolver(table=tb, mode="batched")
+            if name == "d4":
+                sols = solver.solve(c4)
+                st = solver.last_stats
+                lens = [len(s.moves) for s in sols]
+                out["c4_d4"] = {"cubes": len(c4), "cubes_per_s": round(len(c4) / (st.kernel_ms / 1e3), 2),
+                                "kernel_s": round(st.kernel_ms / 1e3, 2),
+                                "lengths_match_reference": lens[:256] == [ref["results"][i]["len"] for i in pick],
+                                "superflip_len": lens[-1]}
+                log(f"C4 on d4: {out['c4_d4']}")
+            times, ok = [], True
+            for c in d20:
+                t = time.perf_counter()
+                ok &= len(solver.solve([c])[0].moves) == 20
+                times.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            sol = solver.solve([sf])[0]
+            sf_s = time.perf_counter() - t
+            out[f"alone_{name}"] = {"distance20_median_s": round(sorted(times)[len(times) // 2], 3),
+                                    "distance20_max_s": round(max(times), 3), "distance20_all_20": ok,
+                                    "superflip_s": round(sf_s, 2), "superflip_len": len(sol.moves)}
+            log(f"alone on {name}: {out[f'alone_{name}']}")
+    finally:
+        td4.close()
+    out["reference"] = ("RTX 5060 Ti (docs/benchmarks.md:75-90): distance-20 median 1.02 s on d4t, 1.00 s on "
+                        "d4; superflip 16.0 s on d4t, 3.2 s on d4")
+    return out
+
+
+def table_load(tb, log) -> dict:
+    """ccs_tables_open on the d4t block file written by ccs_tables_save: once
+    with the file dropped from the page cache (cold), once warm.  The
+    reference quotes 1-5 s per load (README.md:152)."""
+    import tempfile
+
+    from cuda_cube_solver_amd.solver import Tables
+
+    d = Path(tempfile.mkdtemp(prefix="ccs_bench_"))
+    path = d / "prune-ud-d4t-blk.bin"
+    try:
+        t = time.perf_counter()
+        tb.save("block", path)
+        save_s = time.perf_counter() - t
+        fd = os.open(path, os.O_RDONLY)
+        try:
+            os.fsync(fd)
+            os.posix_fadvise(fd, 0, 0, os.POSIX_FADV_DONTNEED)
+        finally:
+            os.close(fd)
+        out = {"bytes": path.stat().st_size, "save_s": round(save_s, 2)}
+        for name in ("cold", "warm"):
+            t = time.perf_counter()
+            t2 = Tables.open(path, "d4t", "block")
+            sec = time.perf_counter() - t
+            t2.close()
+            out[f"{name}_s"] = round(sec, 2)
+            out[f"{name}_gb_s"] = round(out["bytes"] / sec / 1e9, 2)
+        log(f"table load: {out}")
+        return out
+    finally:
+        if path.exists():
+            path.unlink()
+        d.rmdir()
 
 
 def main() -> None:
@@ -131,9 +275,13 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cubes", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--plain", type=int, default=128, help="cubes solved one at a time for the plain figure")
+    ap.add_argument("--plain", type=int, default=-1,
+                    help="workload cubes solved one at a time for the plain figure (-1: all of them)")
     ap.add_argument("--table", default="d4t")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cubes", type=int, default=64, help="workload states the CPU baseline solves")
+    ap.add_argument("--no-hard", action="store_true", help="skip the hard-tail measurements")
+    ap.add_argument("--no-load", action="store_true", help="skip the table load timing")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-probe ceiling measurement")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner; use --chunk-nodes etc.")
     ap.add_argument("--tune-cubes", type=int, default=2048, help="autotuner sample (seeded uniform states, seed 33)")
@@ -161,6 +309,11 @@ def main() -> None:
     build_s = time.perf_counter() - t
     log(f"built {args.table} in {build_s:.1f}s, depth counts {counts}")
     cubes = states.uniform_states(args.cubes, args.seed)
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, host_threads() - 2)
+        log(f"CPU baseline: {args.cpu_cubes} states on {threads} threads ({cpu_model()}), beside the GPU steps")
+        cpu = CpuBaseline(tb, cubes[: args.cpu_cubes], threads)
     knobs = {"chunk_nodes": args.chunk_nodes, "blocks_per_cu": args.blocks_per_cu, "prefix_len": args.prefix_len,
              "deeper_prefix_at": (5e7, 3e9)}
     tuned = None
@@ -197,10 +350,19 @@ def main() -> None:
     value = args.cubes * args.steps / (kernel_ms / 1e3)
     end_to_end = args.cubes * args.steps / total_s
 
+    # the CPU baseline shares the host: it ends before the host-timed plain leg
+    cpu_line = None
+    if cpu is not None:
+        try:
+            cpu_line = cpu.finish(lengths)
+        except Exception as e:  # noqa: BLE001
+            cpu_line = {"error": f"{type(e).__name__}: {e}"}
+        log(f"CPU baseline: {cpu_line}")
     # plain: one cube at a time, per-round launches (the reference's form at batch 1)
     plain_ms = []
     plain_len = []
-    for s in cubes[: args.plain]:
+    n_plain = len(cubes) if args.plain < 0 else args.plain
+    for s in cubes[:n_plain]:
         t = time.perf_counter()
         sol = solver.solve([s], mode="plain")[0]
         plain_ms.append((time.perf_counter() - t) * 1e3)
@@ -239,6 +401,7 @@ def main() -> None:
         "end_to_end_ms_per_step": round(total_s / args.steps * 1e3, 1),
         "batched_over_plain": round(end_to_end / plain_value, 2) if plain_ms else None,
         "lengths_identical": same,
+        "lengths_compared": len(plain_len),
         "length_histogram": {str(L): lengths.count(L) for L in sorted(set(lengths))},
         "probes_per_cube": round(probes / (args.cubes * args.steps)),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -257,6 +420,10 @@ def main() -> None:
 
     if not args.no_gather:
         line["probe_ceiling"] = extra("probe ceiling", lambda: gather_ceiling(tb))
+    if not args.no_hard:
+        line["hard_tail"] = extra("hard tail", lambda: hard_tail(tb, log))
+    if not args.no_load:
+        line["table_load"] = extra("table load", lambda: table_load(tb, log))
     if not args.no_serve:
         from cuda_cube_solver_amd import serve_bench
 
@@ -268,15 +435,17 @@ def main() -> None:
             def one():
                 r = serve_bench.run(tb, qps, max(16, int(qps * args.serve_seconds)), seed=5, hard_pool=hard,
                                     log=log)
-                r.pop("lengths")
+                lens = r.pop("lengths")
+                r["superflips"] = sum(1 for k, c in serve_bench.request_mix(r["requests"], 5, hard)
+                                      if k == "flip" and c.eo == (1,) * 12)
+                r["all_solved"] = min(lens) >= 0
                 return r
 
             r = extra(f"serving at {qps:.0f} QPS", one)
             r["load"] = f"{frac:.0%} of the batched capacity"
             line["serving"].append(r)
-    if not args.no_cpu:
-        line["cpu_baseline"] = extra("CPU baseline", lambda: cpu_baseline(
-            tb, cubes, nodes_total / (args.cubes * args.steps), 20.0))
+    if cpu_line is not None:
+        line["cpu_baseline"] = cpu_line
     print(json.dumps(line), flush=True)
 
 

@@ -904,6 +904,7 @@ __device__ __forceinline__ void end_chunk(const SearchArgs& A, uint32_t c, Searc
 // one in-flight share of the round until its taker releases it, so a round
 // still closes only when all of its work has drained.
 constexpr uint32_t SPLIT_CHECK = 512;  // passes between two looks of a wave (a power of two)
+constexpr uint32_t SERVE_DONATE_LIVE = 4;  // serving: donations only while at most this many cubes are in
 constexpr int SPLIT_MIN_LEFT = 7;
 constexpr unsigned long long SPLIT_MIN_NODES = 2048;
 constexpr int SPLIT_MAX_K = 7;         // deepest donated node: its path and skip byte fill one u64
@@ -1079,7 +1080,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     bool done = false;
 
     for (;;) {
-        if (QUEUE && !SERVE) {
+        if (QUEUE) {
             // Donation: every SPLIT_CHECK passes the wave looks at whether lanes are
             // out of work (one load, a wait of about a microsecond in some
             // milliseconds of passes); if so, each lane deep in a subtree hands the
@@ -1271,14 +1272,17 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                                 (SERVE && sys_load(A.host_stop))) {
                                 done = true;
                             } else {
-                                // out of work: once the wait has lasted, ask for donations
-                                // (batches only: while serving, donated work waits behind new
-                                // rounds in the ring and holds its own round open, which cost
-                                // 13% of the served rate at 80% load) and back off longer, so
-                                // that idle waves poll the queue words far less
+                                // out of work: once the wait has lasted, ask for donations and
+                                // back off longer, so that idle waves poll the queue words far
+                                // less.  A server asks only while few cubes are in it: under
+                                // load, donated work would wait behind new rounds in the ring
+                                // and hold its own round open (13% of the served rate at 80%
+                                // load), while a lone hard cube needs the whole GPU.
                                 unsigned long long now = wall_clock64();
                                 const bool waited = idle_since && now - idle_since > A.backoff_ticks;
-                                if (!SERVE && waited && rank == 0 && !ld_c(&A.q->split_want))
+                                const bool few = !SERVE || (uint32_t)ld_c(&A.q->mb_head) - ld_c(&A.q->n_final) <=
+                                                               SERVE_DONATE_LIVE;
+                                if (few && waited && rank == 0 && !ld_c(&A.q->split_want))
                                     st_c(&A.q->split_want, 1u);
                                 idle = waited ? 1024 : 64;
                                 // watchdog: a lane that finds no work for idle_limit ends the launch

@@ -18,11 +18,11 @@ from .solver import Tables
 
 
 def request_mix(n: int, seed: int, hard_pool: list) -> list:
+    """n (kind, cube) requests: 80% uniform, 15% from hard_pool, 5% flip-heavy
+    (a seeded list whose last state is the superflip, all 12 edges flipped)."""
     rng = states.SplitMix64(seed)
     uni = states.uniform_states(n, seed + 1000)
-    # the superflip (all 12 edges flipped) is not in the mix: the seeded list ends
-    # with it and a random even subset can also be all 12
-    flips = [f for f in states.flip_heavy_states(max(n // 20 + 2, 2), seed + 2000) if f.eo != (1,) * 12]
+    flips = states.flip_heavy_states(max(n // 20 + 2, 2), seed + 2000)
     out = []
     for i in range(n):
         r = rng.below(100)
