@@ -579,6 +579,10 @@ constexpr int T_CHUNK_SHIFT = 59;
 constexpr unsigned long long T_BANDED = 1ull << 62;  // the round visits its items in N_BANDS key bands
 // Serving: a fetching wave admits new cubes while fewer rounds than this are queued.
 constexpr unsigned long long ADMIT_BELOW = 256;
+// Serving: while at most this many cubes are in the server, lanes out of work
+// ask for donated subtrees and chunks aim at 64x the serving chunk size (a
+// lone hard cube then gets the whole GPU, as in a batch).
+constexpr uint32_t SERVE_DONATE_LIVE = 4;
 // Ring entry: (slot mod 2^27) << 37 | round bound << 32 | cube.  The slot part
 // tells a reader that the entry at its head has been published; the bound
 // tells a stale entry (of a round that has since closed) from the cube's live one.
@@ -647,8 +651,9 @@ __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned 
                                         : (unsigned long long)(A.q->n_jobs - ld_c(&A.q->n_final));
     unsigned long long want = A.n_lanes / (live ? live : 1ull);
     want = want < 4096ull ? 4096ull : want;
+    const unsigned long long target = A.serving && live <= SERVE_DONATE_LIVE ? A.chunk_nodes << 6 : A.chunk_nodes;
     uint32_t lg = 0;
-    while (lg < 6 && (per_item << (lg + 1)) <= A.chunk_nodes && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
+    while (lg < 6 && (per_item << (lg + 1)) <= target && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
     return lg;
 }
 
@@ -904,7 +909,6 @@ __device__ __forceinline__ void end_chunk(const SearchArgs& A, uint32_t c, Searc
 // one in-flight share of the round until its taker releases it, so a round
 // still closes only when all of its work has drained.
 constexpr uint32_t SPLIT_CHECK = 512;  // passes between two looks of a wave (a power of two)
-constexpr uint32_t SERVE_DONATE_LIVE = 4;  // serving: donations only while at most this many cubes are in
 constexpr int SPLIT_MIN_LEFT = 7;
 constexpr unsigned long long SPLIT_MIN_NODES = 2048;
 constexpr int SPLIT_MAX_K = 7;         // deepest donated node: its path and skip byte fill one u64

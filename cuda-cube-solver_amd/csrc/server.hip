@@ -66,6 +66,15 @@ struct ccs_server {
     std::atomic<bool> quit{false};
     std::thread pump;
     std::string error;
+
+    // Frees whatever was set up; the serving kernel must have ended (or never started).
+    ~ccs_server() {
+        if (stream) (void)hipStreamDestroy(stream);
+        for (void* p : {(void*)d_jobs, (void*)d_q, (void*)d_ring, (void*)d_stats, (void*)d_splits})
+            if (p) (void)hipFree(p);
+        for (void* p : {(void*)mb, (void*)mb_tail, (void*)done, (void*)host_stop})
+            if (p) (void)hipHostFree(p);
+    }
 };
 
 namespace {
@@ -149,16 +158,18 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     if (o.prefix_len == 0) o.prefix_len = 4;
     if (o.prefix_len < 4 || o.prefix_len > MAX_PLEN) return fail(CCS_EINVAL, "prefix_len must be 4..7");
     if (o.deeper_at[0] == 0 && o.deeper_at[1] == 0) o.deeper_at[0] = 5e7, o.deeper_at[1] = 3e9;
-    // Admission by device memory: each search holds one SearchJob and four ring entries.
+    // Admission by device memory: each search holds one SearchJob and four ring
+    // entries.  (Servers keep no item keys: the banded round order, which they
+    // need, cost more in visits than it saved at serving's cube counts.)
+    for (int p = 0; p <= std::min(o.prefix_len + 2, MAX_PLEN); ++p) {
+        int rc = ensure_prefixes(t, p);
+        if (rc) return rc;
+    }
     const uint64_t per_search = sizeof(SearchJob) + 4 * sizeof(unsigned long long);
     uint64_t budget = o.state_budget_bytes ? o.state_budget_bytes : (64ull << 20);
     int n_slots = (int)std::min<uint64_t>(budget / per_search, 1u << 20);
     if (o.max_inflight > 0) n_slots = std::min(n_slots, o.max_inflight);
     if (n_slots < 1) return fail(CCS_ENOMEM, "state budget holds no search");
-    for (int p = 0; p <= std::min(o.prefix_len + 2, MAX_PLEN); ++p) {
-        int rc = ensure_prefixes(t, p);
-        if (rc) return rc;
-    }
     std::unique_ptr<ccs_server> s(new ccs_server());
     s->t = t;
     s->n_slots = n_slots;
@@ -285,17 +296,7 @@ int ccs_server_stop(ccs_server_t* s) {
     s->quit.store(true);
     if (s->pump.joinable()) s->pump.join();
     hipError_t e = hipStreamSynchronize(s->stream);
-    (void)hipStreamDestroy(s->stream);
-    (void)hipFree(s->d_jobs);
-    (void)hipFree(s->d_q);
-    (void)hipFree(s->d_ring);
-    (void)hipFree(s->d_splits);
-    (void)hipFree(s->d_stats);
-    (void)hipHostFree(s->mb);
-    (void)hipHostFree(s->mb_tail);
-    (void)hipHostFree(s->done);
-    (void)hipHostFree(s->host_stop);
-    delete s;
+    delete s;  // frees the device and pinned memory
     if (e != hipSuccess) return fail(CCS_EDEVICE, std::string("serving kernel: ") + hipGetErrorString(e));
     return CCS_OK;
 }

@@ -52,7 +52,7 @@ def pct(xs, q):
 
 
 def run(tables: Tables, qps: float, n: int, seed: int = 5, hard_pool: list | None = None,
-        state_budget_bytes: int = 64 << 20, log=None, **server_kw) -> dict:
+        state_budget_bytes: int = 0, log=None, **server_kw) -> dict:
     """Offer n requests at `qps` (Poisson arrivals); `server_kw` goes to Server."""
     reqs = request_mix(n, seed, hard_pool or [])
     when = poisson_times(n, qps, seed + 1)

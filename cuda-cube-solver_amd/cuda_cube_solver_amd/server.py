@@ -25,7 +25,7 @@ class Server:
     """``state_budget_bytes`` bounds the device memory of searches in flight
     (admission stops above it); ``max_inflight`` caps them further."""
 
-    def __init__(self, tables: Tables, state_budget_bytes: int = 64 << 20, max_inflight: int = 0,
+    def __init__(self, tables: Tables, state_budget_bytes: int = 0, max_inflight: int = 0,
                  prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0, chunk_nodes: int = 0):
         o = _lib.ServerOpts()
         o.state_budget_bytes = state_budget_bytes

@@ -167,7 +167,9 @@ int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs_solve_opts
 /* ---- serving: continuous batching on a resident table ------------------ */
 
 typedef struct {
-    uint64_t state_budget_bytes; /* device memory for per-search state; admission stops above it */
+    uint64_t state_budget_bytes; /* device memory for per-search state (0: 64 MiB); a search holds its job
+                                    and ring entries (608 B), so the budget sets how many cubes are
+                                    searched at once */
     int32_t max_inflight;        /* cubes searched at once (0: from the budget) */
     int32_t prefix_len;          /* as in ccs_solve_opts_t */
     double deeper_at[2];
