@@ -59,6 +59,13 @@ int tables_setup(ccs_tables* t, const std::string& name) {
     HIP_TRY(hipMemcpy(t->d_classidx, cc.classidx.data(), cc.classidx.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t->d_class_rep, cc.rep.data(), cc.rep.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t->d_class_stab, cc.stab.data(), cc.stab.size() * 2, hipMemcpyHostToDevice));
+    // The work-item prefixes of the default prefix length 4 and its deeper rounds
+    // (lengths 0..6, 62 MB): resident with the table, so that a solve needs no
+    // allocation for them.
+    for (int p = 0; p <= 6; ++p) {
+        int rc = ensure_prefixes(t, p);
+        if (rc) return rc;
+    }
     return CCS_OK;
 }
 

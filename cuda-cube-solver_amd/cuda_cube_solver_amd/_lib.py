@@ -21,6 +21,7 @@ CCS_EFAULT = -14
 CCS_EBUSY = -16
 CCS_EINVAL = -22
 CCS_EDEVICE = -100
+NOALLOC_MAX_CUBES = 256
 FORMAT_BLOCK = 0
 FORMAT_B3 = 1
 TABLE_DEPTHS = 2
@@ -95,6 +96,7 @@ SIGNATURES = {
     "ccs_probe": (I, [P, P, I, I, P, P, P]),
     "ccs_unrank_check": (I, [P, P, I, P]),
     "ccs_solve": (I, [P, P, I, P, P, P]),
+    "ccs_solve_noalloc": (I, [P, P, I, P, P, P]),
     "ccs_server_start": (I, [P, P, P]),
     "ccs_server_submit": (I, [P, P, U64]),
     "ccs_server_poll": (I, [P, P, I, I]),

@@ -164,6 +164,22 @@ typedef struct {
 int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs_solve_opts_t* opts, ccs_result_t* out,
               ccs_solve_stats_t* stats);
 
+/* Most cubes ccs_solve_noalloc takes in one call. */
+#define CCS_NOALLOC_MAX_CUBES 256
+
+/* ccs_solve in the persistent batched form (mode 1, whatever opts->mode
+ * says) with no device allocation: its device state (jobs, round queue, ring,
+ * donated-subtree records, counters; about 0.7 MB at 256 cubes) lives in one
+ * static 1 MiB device array per GPU, and the prefix tables it reads are the
+ * ones every tables handle keeps (lengths 0..6, i.e. prefix_len 4).  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  n above CCS_NOALLOC_MAX_CUBES is refused (CCS_EINVAL).
+ * Calls that overlap on one GPU share that array and race: serialise them.
+ * Kernel time in stats comes from the device clock instead of HIP events. */
+int ccs_solve_noalloc(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs_solve_opts_t* opts, ccs_result_t* out,
+                      ccs_solve_stats_t* stats);
+
 /* ---- serving: continuous batching on a resident table ------------------ */
 
 typedef struct {
