@@ -161,7 +161,11 @@ class CpuBaseline:
             self.error = e
 
     def finish(self, gpu_lengths) -> dict:
-        self.thread.join()
+        t = time.perf_counter()
+        while self.thread.is_alive():
+            self.thread.join(timeout=30.0)
+            if self.thread.is_alive():
+                log(f"CPU baseline still running ({time.perf_counter() - t:.0f} s waited)")
         if self.error is not None:
             raise self.error
         res, bad, wall = self.result

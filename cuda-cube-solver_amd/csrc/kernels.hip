@@ -484,12 +484,17 @@ __global__ void __launch_bounds__(256) encode_b3_kernel(const uint32_t* __restri
 // prefix move) runs between probes, one pass per step in the common cases.
 // The per-lane DFS stack (parent bounds, moves) lives in LDS, lane-major.
 //
-// h and child values are packed 6 x 5 bits in one register.
+// h and child values are packed 6 x 4 bits in one register.  Table values
+// never exceed 15 (the deepest coset table reaches 14), and a value clamped to
+// 15 would still be a lower bound; a stack level is then one u32: the bounds
+// in bits 0..23 and the move in bits 24..28.
 
-__device__ __forceinline__ uint32_t hget(uint32_t hp, int j) { return (hp >> (5 * j)) & 31u; }
+__device__ __forceinline__ uint32_t hget(uint32_t hp, int j) { return (hp >> (4 * j)) & 15u; }
 __device__ __forceinline__ uint32_t hset(uint32_t hp, int j, uint32_t v) {
-    return (hp & ~(31u << (5 * j))) | (v << (5 * j));
+    return (hp & ~(15u << (4 * j))) | (min(v, 15u) << (4 * j));
 }
+constexpr uint32_t STK_H = 0xFFFFFFu;  // stack word: packed bounds
+__device__ __forceinline__ int stk_move(uint32_t w) { return (int)(w >> 24); }
 // Axes whose first move is excluded: an inverse frame at the budget
 // (kernels.cu:750-763).
 __device__ __forceinline__ uint32_t excl_axes(uint32_t hp, uint32_t budget) {
@@ -502,9 +507,9 @@ __device__ __forceinline__ uint32_t axis_mask(uint32_t axes) {
 // All three forward (or inverse) frames at the budget rule out every last
 // (first) move (kernels.cu:765-774).
 __device__ __forceinline__ bool all_tight(uint32_t cp, uint32_t budget) {
-    if (budget == 0) return false;
-    uint32_t b3 = budget | (budget << 5) | (budget << 10);
-    return (cp & 0x7FFFu) == b3 || ((cp >> 15) & 0x7FFFu) == b3;
+    if (budget == 0 || budget > 15u) return false;  // no value reaches 16
+    uint32_t b3 = budget * 0x111u;
+    return (cp & 0xFFFu) == b3 || ((cp >> 12) & 0xFFFu) == b3;
 }
 // Frame probe order (kernels.cu:1398-1410 orders all six frames: at the budget
 // edge, one below, the rest).  Here a step probes one forward and one inverse
@@ -808,14 +813,14 @@ __device__ void release_items(const SearchArgs& A, uint32_t c, uint32_t n) {
 }
 
 __device__ __forceinline__ void report_solution(const SearchArgs& A, SearchJob* J, bool queue, int g,
-                                                const uint8_t* pth, uint32_t stride) {
+                                                const uint32_t* stk, uint32_t stride) {
     if (atomicCAS(&J->found, 0u, 1u) != 0u) return;
     st_c(&J->length, (uint32_t)g);
     uint32_t* sol = reinterpret_cast<uint32_t*>(J->solution);
     for (int w = 0; w < MAX_BOUND / 4; ++w) {
         uint32_t word = 0;
         for (int b = 0; b < 4; ++b)
-            if (4 * w + b < g) word |= (uint32_t)pth[(4 * w + b) * stride] << (8 * b);
+            if (4 * w + b < g) word |= (uint32_t)stk_move(stk[(4 * w + b) * stride]) << (8 * b);
         st_c(&sol[w], word);
     }
     if (queue) {
@@ -919,16 +924,16 @@ constexpr int SPLIT_MAX_K = 7;         // deepest donated node: its path and ski
 // caller clears them).  Out of line, like take_split.
 __device__ __attribute__((noinline)) int donate(QueueCtl* q, SplitRec* splits, const CubeLUT& L, SearchJob* J, uint32_t c,
                                                 uint32_t bound, int g, int m, uint32_t todo, int root, int phase,
-                                                uint32_t found_seen, unsigned long long nodes, const uint32_t* hst,
-                                                const uint8_t* pth, uint32_t nt) {
+                                                uint32_t found_seen, unsigned long long nodes, const uint32_t* stk,
+                                                uint32_t nt) {
     int k = -1;
     uint32_t p = 0;
     if (phase == 3 && !found_seen && nodes >= SPLIT_MIN_NODES) {
         int d = root;
         for (; d < g && d <= SPLIT_MAX_K && (int)bound - d >= SPLIT_MIN_LEFT; ++d) {
-            const uint32_t pd = pth[d * nt];
-            const int ld = d > 0 ? (int)pth[(d - 1) * nt] : -1;
-            const uint32_t rem = L.amask[ld + 1] & ~axis_mask(excl_axes(hst[d * nt], bound - (uint32_t)d)) &
+            const uint32_t pd = (uint32_t)stk_move(stk[d * nt]);
+            const int ld = d > 0 ? stk_move(stk[(d - 1) * nt]) : -1;
+            const uint32_t rem = L.amask[ld + 1] & ~axis_mask(excl_axes(stk[d * nt] & STK_H, bound - (uint32_t)d)) &
                                  ~((2u << pd) - 1u);
             if (rem) {
                 k = d;
@@ -966,7 +971,7 @@ __device__ __attribute__((noinline)) int donate(QueueCtl* q, SplitRec* splits, c
     const unsigned long long slot = (((unsigned long long)bhi << 32) | blo) + rank;
     SplitRec* R = &splits[slot & (SPLIT_RING - 1u)];
     unsigned long long path = (unsigned long long)(p + 1u) << (8 * k);
-    for (int j = 0; j < k; ++j) path |= (unsigned long long)pth[j * nt] << (8 * j);
+    for (int j = 0; j < k; ++j) path |= (unsigned long long)stk_move(stk[j * nt]) << (8 * j);
     atomicAdd(&J->ticket, T_INF_ONE);  // the record's share of the round, released by its taker
     st_c(&R->path, path);
     st_c(&R->cube, c);
@@ -1031,14 +1036,13 @@ __device__ __forceinline__ Frame load_frame_c(const Frame* p) {
 template <int K, int F, bool QUEUE, bool SERVE>
 __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kernel(SearchArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS: CubeLUT, ClassLUT, then the lanes' stacks (u32 bounds, then u8 moves), lane-major
+    // LDS: CubeLUT, ClassLUT, then the lanes' stacks (one u32 per level), lane-major
     CubeLUT& L = *reinterpret_cast<CubeLUT*>(smem);
     ClassLUT& CL = *reinterpret_cast<ClassLUT*>(smem + sizeof(CubeLUT));
     constexpr uint32_t nt = SEARCH_WG;  // the host launches exactly this many lanes
     const uint32_t tid = threadIdx.x;
     unsigned char* stk = smem + sizeof(CubeLUT) + sizeof(ClassLUT);
-    uint32_t* hst = reinterpret_cast<uint32_t*>(stk) + tid;
-    uint8_t* pth = reinterpret_cast<uint8_t*>(reinterpret_cast<uint32_t*>(stk) + STACK_DEPTH * nt) + tid;
+    uint32_t* stw = reinterpret_cast<uint32_t*>(stk) + tid;  // this lane's stack words, STACK_DEPTH of them
     {
         const uint4* s = reinterpret_cast<const uint4*>(A.class_lut);
         uint4* d = reinterpret_cast<uint4*>(&CL);
@@ -1092,7 +1096,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             // the rest.
             wtick = __builtin_amdgcn_readfirstlane(wtick + 1u);
             if ((wtick & (SPLIT_CHECK - 1u)) == 0u && __builtin_amdgcn_readfirstlane(ld_c(&A.q->split_want))) {
-                const int nr = donate(A.q, A.splits, L, J, c, bound, g, m, todo, root, phase, found_seen, nodes, hst, pth, nt);
+                const int nr = donate(A.q, A.splits, L, J, c, bound, g, m, todo, root, phase, found_seen, nodes, stw, nt);
                 if (nr > g) todo = 0;
                 if (nr >= 0) root = nr;
             }
@@ -1120,8 +1124,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 if (!pruned && !all_tight(cp, bound - (uint32_t)g - 1u)) {  // survived: push
                     mv = m;
                     msel = 0;
-                    hst[g * nt] = hp;
-                    pth[g * nt] = (uint8_t)m;
+                    stw[g * nt] = hp | ((uint32_t)m << 24);
                     hp = cp;
                     ++g;
                     if ((uint32_t)g == bound) {
@@ -1408,11 +1411,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     if (SERVE) {  // the slot may have held another cube: read past the L2
                         static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame_c(&J->frame[j]); });
 #pragma unroll
-                        for (int j = 0; j < 6; ++j) hp |= ld_c(&J->h[j]) << (5 * j);
+                        for (int j = 0; j < 6; ++j) hp |= min(ld_c(&J->h[j]), 15u) << (4 * j);
                     } else {
                         static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame(&J->frame[j]); });
 #pragma unroll
-                        for (int j = 0; j < 6; ++j) hp |= J->h[j] << (5 * j);
+                        for (int j = 0; j < 6; ++j) hp |= min(J->h[j], 15u) << (4 * j);
                     }
                     g = 0;
                     ord = 0;
@@ -1430,11 +1433,12 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             }
             if (pop) {
                 --g;
-                const int pm = pth[g * nt];
+                const uint32_t w = stw[g * nt];
+                const int pm = stk_move(w);
                 mv = pm;
                 msel = 1;
-                hp = hst[g * nt];
-                const int last = g > 0 ? (int)pth[(g - 1) * nt] : -1;
+                hp = w & STK_H;
+                const int last = g > 0 ? stk_move(stw[(g - 1) * nt]) : -1;
                 todo = L.amask[last + 1] & ~axis_mask(excl_axes(hp, bound - (uint32_t)g)) & ~((2u << pm) - 1u);
                 ord = 0;
                 if (phase == 1 && g == tgt) phase = 2;
@@ -1491,7 +1495,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         }
         if (mv >= 0) {
             static_for<0, 6>([&](auto j) { frame_move(L, f.template at<j>(), j, msel, mv); });
-            if (leaf && is_solved(f.f0)) report_solution(A, J, QUEUE, g, pth, nt);
+            if (leaf && is_solved(f.f0)) report_solution(A, J, QUEUE, g, stw, nt);
         }
         if (eval) {
             // ---- one step of child m: its next forward and next inverse frame,
@@ -1607,7 +1611,7 @@ void* search_fn(int F, int form) {
 }  // namespace
 
 size_t search_smem_bytes(int block) {
-    return sizeof(CubeLUT) + sizeof(ClassLUT) + (size_t)block * (STACK_DEPTH * 4 + STACK_DEPTH);
+    return sizeof(CubeLUT) + sizeof(ClassLUT) + (size_t)block * STACK_DEPTH * 4;
 }
 
 hipError_t search_occupancy(int K, int F, int form, int block, int* blocks_per_cu) {

@@ -209,3 +209,27 @@ def test_b3_only_build_matches_the_both_build(td4t_depths, table_hashes):
         assert h.hexdigest() == table_hashes["d4t"]["b3_sha256"]
     finally:
         tb.close()
+
+
+@pytest.mark.parametrize("kind", ["s4", "d4f", "a4t", "a4"])
+def test_other_presets_are_bit_exact(kind, table_hashes, reference_results):
+    """The remaining presets' kinds (s4-b3, d4f-b3, a4t, a4t-b3, a4, a4-b3):
+    both files equal the oracle BFS's SHA-256 and the depth counts equal the
+    reference's (research/subgroups/results.jsonl)."""
+    import hashlib
+
+    if kind not in table_hashes:
+        pytest.skip(f"no oracle hashes recorded for {kind}")
+    want = table_hashes[kind]
+    tb, counts = Tables.build(kind, "both")
+    try:
+        assert counts == want["depth_counts"] == reference_results["depth_counts"][kind]["depth_counts"]
+        for fmt, key in (("block", "blk"), ("b3", "b3")):
+            n = tb.nbytes(fmt)
+            assert n == want[f"{key}_bytes"]
+            h = hashlib.sha256()
+            for off in range(0, n, 1 << 28):
+                h.update(tb.read(fmt, off, min(1 << 28, n - off)))
+            assert h.hexdigest() == want[f"{key}_sha256"], f"{kind} {fmt}"
+    finally:
+        tb.close()
