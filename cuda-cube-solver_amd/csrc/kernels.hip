@@ -1039,7 +1039,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // LDS: CubeLUT, ClassLUT, then the lanes' stacks (one u32 per level), lane-major
     CubeLUT& L = *reinterpret_cast<CubeLUT*>(smem);
     ClassLUT& CL = *reinterpret_cast<ClassLUT*>(smem + sizeof(CubeLUT));
-    constexpr uint32_t nt = SEARCH_WG;  // the host launches exactly this many lanes
+    const uint32_t nt = blockDim.x;  // lanes of the workgroup (a multiple of 64, at most SEARCH_WG)
     const uint32_t tid = threadIdx.x;
     unsigned char* stk = smem + sizeof(CubeLUT) + sizeof(ClassLUT);
     uint32_t* stw = reinterpret_cast<uint32_t*>(stk) + tid;  // this lane's stack words, STACK_DEPTH of them
@@ -1505,7 +1505,9 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             // table misses overlap
             __builtin_amdgcn_s_setprio(2);
             bool hasF = remF != 0u, hasI = remI != 0u;
-            if (first) {  // a child's first step probes only its top frame, where most are pruned
+            // a child's first step probes only its top frame, where most are pruned
+            // (with one probe per step, every step probes only the top remaining frame)
+            if (first || A.one_probe) {
                 if ((ctz(remF | 0x10000u) >> 2) <= (ctz(remI | 0x10000u) >> 2)) hasI = false;
                 else hasF = false;
                 first = false;

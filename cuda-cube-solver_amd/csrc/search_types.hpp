@@ -134,6 +134,7 @@ struct SearchArgs {
     int max_bound;
     unsigned long long deeper0, deeper1;
     unsigned int poll_mask;      // found flag polled every poll_mask + 1 picks
+    int one_probe;               // 1: one table probe per step (the top remaining frame); 0: a forward and an inverse
     unsigned long long idle_limit;  // persistent form: device-clock ticks a lane may wait for work
     unsigned long long backoff_ticks; // persistent form: idle ticks after which a lane polls less often
     unsigned long long chunk_nodes; // persistent form: target nodes per fetched chunk of items

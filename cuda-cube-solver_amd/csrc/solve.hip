@@ -76,6 +76,7 @@ void fill_opts(const ccs_solve_opts_t* in, ccs_solve_opts_t& o) {
     if (o.block == 0) o.block = SEARCH_WG;
     if (o.max_bound == 0) o.max_bound = MAX_BOUND;
     if (o.poll_interval == 0) o.poll_interval = 64;
+    if (o.probes_per_step == 0) o.probes_per_step = 2;
     if (o.deeper_at[0] == 0 && o.deeper_at[1] == 0) o.deeper_at[0] = 5e7, o.deeper_at[1] = 3e9;
 }
 
@@ -83,7 +84,9 @@ int check_opts(const ccs_solve_opts_t& o) {
     if (o.mode != 0 && o.mode != 1) return fail(CCS_EINVAL, "mode must be 0 (plain) or 1 (persistent)");
     if (o.prefix_len < 4 || o.prefix_len > MAX_PLEN) return fail(CCS_EINVAL, "prefix_len must be 4..7");
     if (o.max_bound < 1 || o.max_bound > MAX_BOUND) return fail(CCS_EINVAL, "max_bound must be 1..24");
-    if (o.block != SEARCH_WG) return fail(CCS_EINVAL, "block must be " + std::to_string(SEARCH_WG));
+    if (o.block < 64 || o.block > SEARCH_WG || o.block % 64)
+        return fail(CCS_EINVAL, "block must be a multiple of 64 up to " + std::to_string(SEARCH_WG));
+    if (o.probes_per_step != 1 && o.probes_per_step != 2) return fail(CCS_EINVAL, "probes_per_step must be 1 or 2");
     if (o.poll_interval & (o.poll_interval - 1)) return fail(CCS_EINVAL, "poll_interval must be a power of two");
     return CCS_OK;
 }
@@ -318,6 +321,7 @@ void fill_args(ccs_tables* t, const ccs_solve_opts_t& o, int grid, SearchJob* d_
     A.deeper0 = to_u64(o.deeper_at[0]);
     A.deeper1 = to_u64(o.deeper_at[1]);
     A.poll_mask = (unsigned)o.poll_interval - 1u;
+    A.one_probe = o.probes_per_step == 1;
     A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : DEFAULT_CHUNK_NODES;
     A.n_lanes = (unsigned long long)grid * (unsigned long long)o.block;
     A.stats = d_stats;

@@ -52,7 +52,8 @@ class KindInfo(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("prefix_len", ctypes.c_int32), ("deeper_at", ctypes.c_double * 2),
                 ("block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("max_bound", ctypes.c_int32),
-                ("poll_interval", ctypes.c_int32), ("chunk_nodes", ctypes.c_int32)]
+                ("poll_interval", ctypes.c_int32), ("chunk_nodes", ctypes.c_int32),
+                ("probes_per_step", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):

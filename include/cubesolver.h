@@ -131,11 +131,14 @@ typedef struct {
     int32_t mode;           /* 0: plain (per-round launches, the reference's form); 1: persistent batched */
     int32_t prefix_len;     /* work-item prefix length, >= 4 (solver.py:119, default 4) */
     double deeper_at[2];    /* longer prefixes after rounds of this many nodes (solver.py:123: 5e7, 3e9) */
-    int32_t block;          /* workgroup size (0: 768, the only size the kernels are built for) */
+    int32_t block;          /* lanes per workgroup, a multiple of 64 up to 768 (0: 768); one workgroup runs per CU */
     int32_t blocks_per_cu;  /* persistent grid per CU (0: from occupancy) */
     int32_t max_bound;      /* give up above this bound (0: 24); searches stop at 20, the longest optimal solution of any legal cube */
     int32_t poll_interval;  /* picks between polls of the found flag, power of two (0: 64) */
     int32_t chunk_nodes;    /* persistent form: target nodes per chunk of items a lane takes (0: 262144) */
+    int32_t probes_per_step; /* table probes a lane keeps in flight per step, 1 or 2 (0: 2): a child's
+                               first step probes its top frame alone, later steps one forward and one
+                               inverse frame (2) or the top remaining frame (1); nodes are the same */
 } ccs_solve_opts_t;
 
 typedef struct {
