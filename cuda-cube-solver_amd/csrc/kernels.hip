@@ -588,6 +588,16 @@ constexpr unsigned long long ADMIT_BELOW = 256;
 // ask for donated subtrees and chunks aim at 64x the serving chunk size (a
 // lone hard cube then gets the whole GPU, as in a batch).
 constexpr uint32_t SERVE_DONATE_LIVE = 4;
+// Serving: a cube's turn at the head of the ring lasts 1 + age / age_ticks
+// fetches, at most this many, so that cubes that have waited longest get a
+// larger share of the lanes (the round robin's plain equal shares made every
+// cube's latency grow with the number of cubes in the server).
+constexpr uint32_t SERVE_MAX_CREDIT = 16;
+__device__ __forceinline__ uint32_t serve_credit(const SearchArgs& A, const SearchJob* J) {
+    const unsigned long long age = (unsigned long long)wall_clock64() - ld_c(&J->t_admit);
+    const unsigned long long c = 1ull + age / (A.age_ticks ? A.age_ticks : 1ull);
+    return (uint32_t)(c < SERVE_MAX_CREDIT ? c : SERVE_MAX_CREDIT);
+}
 // Ring entry: (slot mod 2^27) << 37 | round bound << 32 | cube.  The slot part
 // tells a reader that the entry at its head has been published; the bound
 // tells a stale entry (of a round that has since closed) from the cube's live one.
@@ -732,6 +742,7 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
     uint32_t clog = chunk_log_for(A, nodes, A.pre_n[plen] * (banded ? N_BANDS : 1u));
     st_c(&J->nodes, 0ull);
     st_c(&J->bound, nb);
+    if (A.serving) st_c(&J->credit, serve_credit(A, J));
     complete();
     unsigned long long cur = ld_c(&J->ticket);
     for (;;) {  // keep the in-flight count of late fetchers, reset the item cursor
@@ -874,6 +885,7 @@ __device__ bool admit_one(const SearchArgs& A, const CubeLUT& L) {
     st_c(&J->length, 0u);
     st_c(&J->n_rounds, 0u);
     st_c(&J->t_admit, (unsigned long long)wall_clock64());
+    st_c(&J->credit, 1u);
     st_c(&J->nodes, 0ull);
     st_c(&J->status, 0u);
     st_c(&J->found, 0u);
@@ -1261,10 +1273,15 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                                 // gets about its share of the lanes and its round is searched
                                 // roughly in item order (a final round then stops near where a
                                 // sequential search would).  Only the wave that advances the head
-                                // re-queues the entry, and only while items remain.
-                                if (atomicCAS(&A.q->head, h, h + 1) == h &&
-                                    t_next(old) + ((unsigned long long)cnt << clog) < t_visits(A, old))
+                                // re-queues the entry, and only while items remain.  A server's
+                                // cube keeps the head for its turn's credit of fetches.
+                                bool move_on = true;
+                                if (SERVE) move_on = (int)atomicSub(&Je->credit, 1u) <= 1;
+                                if (move_on && atomicCAS(&A.q->head, h, h + 1) == h &&
+                                    t_next(old) + ((unsigned long long)cnt << clog) < t_visits(A, old)) {
+                                    if (SERVE) st_c(&Je->credit, serve_credit(A, Je));
                                     ring_push(A, eb, ce);
+                                }
                                 cc = ce;
                                 break;
                             }

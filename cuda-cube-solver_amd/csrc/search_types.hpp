@@ -51,6 +51,7 @@ struct alignas(64) SearchJob {
     uint32_t round_bound[MAX_BOUND];
     unsigned long long round_nodes[MAX_BOUND];
     uint32_t band_thr[4];        // persistent form: item-key bounds of the round's bands (N_BANDS - 1 bytes)
+    uint32_t credit;             // serving: fetches the cube keeps the ring's head for, on its turn
 };
 
 // Control block of the persistent form.
@@ -148,6 +149,7 @@ struct SearchArgs {
     unsigned long long done_size;
     const unsigned int* host_stop;       // pinned host: 1 = stop
     unsigned long long mb_poll_ticks;    // device-clock ticks between two reads of the mailbox tail
+    unsigned long long age_ticks;        // serving: a cube's turn at the head grows by one fetch per this age
     SplitRec* splits;                    // persistent form: SPLIT_RING donated subtrees
     // persistent batches: per cube and prefix length, two buffers of one key
     // byte per item (1 + the sum of the six bounds at the prefix's end; 0

@@ -26,7 +26,7 @@ class Server:
     (admission stops above it); ``max_inflight`` caps them further."""
 
     def __init__(self, tables: Tables, state_budget_bytes: int = 0, max_inflight: int = 0,
-                 prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0, chunk_nodes: int = 0):
+                 prefix_len: int = 4, deeper_prefix_at=(0.0, 0.0), blocks_per_cu: int = 0, chunk_nodes: int = 0):
         o = _lib.ServerOpts()
         o.state_budget_bytes = state_budget_bytes
         o.max_inflight = max_inflight
