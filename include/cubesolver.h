@@ -193,7 +193,7 @@ typedef struct {
     int32_t prefix_len;          /* as in ccs_solve_opts_t */
     double deeper_at[2];         /* as for ccs_solve, but 0, 0 means 1e6, 1e8: items of long rounds get
                                     shorter sooner than in ccs_solve, so that no single item holds a
-                                    cube back for long on a shared GPU) */
+                                    cube back for long on a shared GPU */
     int32_t blocks_per_cu;
     int32_t chunk_nodes;         /* target nodes per chunk of items a lane takes (0: 16384); a
                                     lane covers ~40k nodes/s, so this bounds how long one chunk
