@@ -1098,8 +1098,20 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     int hold = -1, hfail = -1;
     uint32_t atick = 0;  // serving: fetches since the mailbox was last checked
     bool done = false;
+#ifdef CCS_SEARCH_STATS
+    // Diagnostics (a build with -DCCS_SEARCH_STATS only; see SearchArgs::stats):
+    // per lane, probes and prunes per frame and frame updates; per wave, steps,
+    // a histogram of the lanes that probe in a step, and the shader-clock cycles
+    // of a step's parts.
+    unsigned long long d_look[6] = {}, d_prune[6] = {}, d_upd = 0;
+    unsigned long long d_steps = 0, d_hist[9] = {}, d_cyc[4] = {};
+    long long d_t0 = 0, d_t1 = 0;
+#endif
 
     for (;;) {
+#ifdef CCS_SEARCH_STATS
+        d_t0 = clock64();
+#endif
         if (QUEUE) {
             // Donation: every SPLIT_CHECK passes the wave looks at whether lanes are
             // out of work (one load, a wait of about a microsecond in some
@@ -1510,10 +1522,23 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
+#ifdef CCS_SEARCH_STATS
+        d_t1 = clock64();
+        d_cyc[0] += (unsigned long long)(d_t1 - d_t0);  // the pass: bookkeeping, fetches, picks
+        d_t0 = d_t1;
+        ++d_steps;
+        ++d_hist[((uint32_t)__popcll(__ballot(eval)) + 7u) >> 3];
+        d_upd += mv >= 0;
+#endif
         if (mv >= 0) {
             static_for<0, 6>([&](auto j) { frame_move(L, f.template at<j>(), j, msel, mv); });
             if (leaf && is_solved(f.f0)) report_solution(A, J, QUEUE, g, stw, nt);
         }
+#ifdef CCS_SEARCH_STATS
+        d_t1 = clock64();
+        d_cyc[1] += (unsigned long long)(d_t1 - d_t0);  // the frame updates
+        d_t0 = d_t1;
+#endif
         if (eval) {
             // ---- one step of child m: its next forward and next inverse frame,
             // both table reads in flight together ----
@@ -1540,6 +1565,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             const uint8_t* bF = probe_block<K, F>(L, pkF, table, tf, jF);
             const uint8_t* bI = probe_block<K, F>(L, pkI, table, ti, jI);
             __builtin_amdgcn_s_setprio(0);
+#ifdef CCS_SEARCH_STATS
+            d_t1 = clock64();
+            d_cyc[2] += (unsigned long long)(d_t1 - d_t0);  // the probes' addresses
+            d_t0 = d_t1;
+#endif
             uint32_t vf = hasF ? block_value<F>(L, bF, jF) : 0u;
             const uint32_t vi = hasI ? block_value<F>(L, bI, jI) : 0u;
             probes += (uint32_t)hasF + (uint32_t)hasI;
@@ -1556,12 +1586,40 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             }
             pruned = pr;
             cdone = pr || (remF | remI) == 0u;
+#ifdef CCS_SEARCH_STATS
+            static_for<0, 3>([&](auto j) {
+                d_look[j] += hasF && kf == (uint32_t)j;
+                d_look[3 + j] += hasI && ki == (uint32_t)j;
+                d_prune[j] += hasF && kf == (uint32_t)j && (uint32_t)g + 1u + vf > bound;
+                d_prune[3 + j] += hasI && ki == (uint32_t)j && (uint32_t)g + 1u + vi > bound;
+            });
+            d_t1 = clock64();
+            d_cyc[3] += (unsigned long long)(d_t1 - d_t0);  // waiting for the table reads, and the values
+#endif
         }
     }
     if (A.stats) {
         unsigned long long p = probes;
         for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o);
         if (lane_id() == 0) atomicAdd(&A.stats[0], p);
+#ifdef CCS_SEARCH_STATS
+        // lanes' counters summed over the wave; the wave's (all lanes equal) once
+        auto wave_sum = [&](unsigned long long v) {
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            return v;
+        };
+        for (int i = 0; i < 6; ++i) {
+            const unsigned long long a = wave_sum(d_look[i]), b = wave_sum(d_prune[i]);
+            if (lane_id() == 0) atomicAdd(&A.stats[STAT_DIAG + i], a), atomicAdd(&A.stats[STAT_DIAG + 6 + i], b);
+        }
+        const unsigned long long u = wave_sum(d_upd);
+        if (lane_id() == 0) {
+            atomicAdd(&A.stats[STAT_DIAG + 12], d_steps);
+            atomicAdd(&A.stats[STAT_DIAG + 13], u);
+            for (int i = 0; i < 9; ++i) atomicAdd(&A.stats[STAT_DIAG + 14 + i], d_hist[i]);
+            for (int i = 0; i < 4; ++i) atomicAdd(&A.stats[STAT_DIAG + 23 + i], d_cyc[i]);
+        }
+#endif
     }
 }
 

@@ -17,6 +17,9 @@ constexpr unsigned long long DEFAULT_CHUNK_NODES = 262144;
 // from waiting seconds on a single lane.
 constexpr unsigned long long SERVE_CHUNK_NODES = 16384;
 constexpr int MAX_PLEN = 7;     // longest work-item prefix
+constexpr int STAT_WORDS = 40;  // device statistics words of a search launch
+constexpr int STAT_DIAG = 8;    // first diagnostics word (27 of them)
+constexpr int DIAG_WORDS = 27;
 // Persistent batches of at least BAND_MIN_CUBES cubes: item keys are kept for
 // rounds with prefixes of at most KEY_MAX_PLEN moves, and a long round (at
 // least BAND_MIN_NODES_PER_ITEM nodes per item in the round before) whose
@@ -158,7 +161,8 @@ struct SearchArgs {
     uint32_t* keys[MAX_PLEN + 1];
     unsigned long long key_words[MAX_PLEN + 1];
     uint32_t* key_hist;
-    // statistics: [0] probes, [1] nodes
+    // statistics (STAT_WORDS words): [0] probes; from STAT_DIAG on, the
+    // diagnostics of a -DCCS_SEARCH_STATS build (ccs_solve_stats_t::diag)
     unsigned long long* stats;
 };
 

@@ -183,7 +183,7 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     HIP_TRY(hipMalloc(&s->d_jobs, sizeof(SearchJob) * (size_t)n_slots));
     HIP_TRY(hipMalloc(&s->d_q, sizeof(QueueCtl)));
     HIP_TRY(hipMalloc(&s->d_ring, sizeof(unsigned long long) * ring_size));
-    HIP_TRY(hipMalloc(&s->d_stats, 4 * sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc(&s->d_stats, STAT_WORDS * sizeof(unsigned long long)));
     HIP_TRY(hipMalloc(&s->d_splits, sizeof(SplitRec) * SPLIT_RING));
     HIP_TRY(hipMemset(s->d_splits, 0, sizeof(SplitRec) * SPLIT_RING));
     const unsigned flags = hipHostMallocCoherent | hipHostMallocMapped;
@@ -209,7 +209,7 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     QueueCtl q{};
     q.n_jobs = 0xFFFFFFFFu;
     HIP_TRY(hipMemcpy(s->d_q, &q, sizeof q, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(s->d_stats, 0, 4 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(s->d_stats, 0, STAT_WORDS * sizeof(unsigned long long)));
     s->free_slots.reserve(n_slots);
     for (int i = n_slots - 1; i >= 0; --i) s->free_slots.push_back(i);
     s->slot_info.resize(n_slots);

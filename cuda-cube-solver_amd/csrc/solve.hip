@@ -281,6 +281,15 @@ int solve_persistent(ccs_tables* t, const ccs_solve_opts_t& o, SearchArgs& A, in
     return CCS_OK;
 }
 
+// The diagnostics words of a launch into the caller's statistics.
+void copy_diag(const unsigned long long* hs, ccs_solve_stats_t* stats) {
+    static_assert(DIAG_WORDS <= (int)(sizeof stats->diag / sizeof stats->diag[0]), "diag words");
+#ifdef CCS_SEARCH_STATS
+    stats->diag_built = 1;
+#endif
+    for (int i = 0; i < DIAG_WORDS; ++i) stats->diag[i] = hs[STAT_DIAG + i];
+}
+
 // Verify every solution (solver.py:317-319) and fill the results.
 int finish_results(const uint8_t* cubes, int n, const std::vector<SearchJob>& jobs, const std::vector<int>& solved0,
                    const std::vector<double>& done_ms, ccs_result_t* out) {
@@ -362,7 +371,7 @@ int scratch_address(uint8_t** out) {
 size_t noalloc_bytes(int n) {
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     return up(sizeof(Frame) * 6ull * n) + up(4ull * 6 * n) + up(sizeof(SearchJob) * (size_t)n) +
-           up(4 * sizeof(unsigned long long)) + up(sizeof(QueueCtl)) + up(8ull * (2ull * n + 1024)) + up(16) +
+           up(STAT_WORDS * sizeof(unsigned long long)) + up(sizeof(QueueCtl)) + up(8ull * (2ull * n + 1024)) + up(16) +
            up(sizeof(SplitRec) * SPLIT_RING);
 }
 
@@ -400,8 +409,8 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     SearchJob* d_jobs = nullptr;
     unsigned long long* d_stats = nullptr;
     HIP_TRY(buf.alloc(&d_jobs, sizeof(SearchJob) * (size_t)n));
-    HIP_TRY(buf.alloc(&d_stats, 4 * sizeof(unsigned long long)));
-    HIP_TRY(zero_async(d_stats, 4 * sizeof(unsigned long long), t->stream));
+    HIP_TRY(buf.alloc(&d_stats, STAT_WORDS * sizeof(unsigned long long)));
+    HIP_TRY(zero_async(d_stats, STAT_WORDS * sizeof(unsigned long long), t->stream));
     double kernel_ms = 0;
     uint32_t launches = 0;
     SearchArgs A;
@@ -491,9 +500,10 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
         if (rc) return rc;
     }
     if (stats) {
-        unsigned long long hs[4];
+        unsigned long long hs[STAT_WORDS];
         HIP_TRY(hipMemcpy(hs, d_stats, sizeof hs, hipMemcpyDeviceToHost));
         stats->probes = hs[0];
+        copy_diag(hs, stats);
         stats->nodes = 0;  // children evaluated plus prefix moves replayed, every round of every cube
         for (int i = 0; i < n; ++i) stats->nodes += out[i].nodes;
         stats->kernel_ms = kernel_ms;
@@ -541,8 +551,8 @@ extern "C" int ccs_solve_noalloc(ccs_tables_t* t, const uint8_t* cubes, int n, c
     SearchJob* d_jobs = nullptr;
     unsigned long long* d_stats = nullptr;
     HIP_TRY(mem.alloc(&d_jobs, sizeof(SearchJob) * (size_t)n));
-    HIP_TRY(mem.alloc(&d_stats, 4 * sizeof(unsigned long long)));
-    HIP_TRY(zero_async(d_stats, 4 * sizeof(unsigned long long), t->stream));
+    HIP_TRY(mem.alloc(&d_stats, STAT_WORDS * sizeof(unsigned long long)));
+    HIP_TRY(zero_async(d_stats, STAT_WORDS * sizeof(unsigned long long), t->stream));
     SearchArgs A;
     fill_args(t, o, grid, d_jobs, d_stats, A);
     double kernel_ms = 0;
@@ -551,9 +561,10 @@ extern "C" int ccs_solve_noalloc(ccs_tables_t* t, const uint8_t* cubes, int n, c
     rc = solve_persistent(t, o, A, grid, n, jobs, solved0, mem, false, t_start, out, done_ms, kernel_ms, launches);
     if (rc) return rc;
     if (stats) {
-        unsigned long long hs[4];
+        unsigned long long hs[STAT_WORDS];
         HIP_TRY(hipMemcpy(hs, d_stats, sizeof hs, hipMemcpyDeviceToHost));
         stats->probes = hs[0];
+        copy_diag(hs, stats);
         for (int i = 0; i < n; ++i) stats->nodes += out[i].nodes;
         stats->kernel_ms = kernel_ms;
         stats->launches = launches;

@@ -64,7 +64,8 @@ class Result(ctypes.Structure):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("probes", ctypes.c_uint64), ("nodes", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
-                ("launches", ctypes.c_uint32), ("wall_ms", ctypes.c_double)]
+                ("launches", ctypes.c_uint32), ("wall_ms", ctypes.c_double), ("diag_built", ctypes.c_int32),
+                ("diag", ctypes.c_uint64 * 32)]
 
 
 class ServerOpts(ctypes.Structure):

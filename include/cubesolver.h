@@ -157,6 +157,18 @@ typedef struct {
     double kernel_ms;             /* time inside search kernels (HIP events) */
     uint32_t launches;
     double wall_ms;
+    /* Diagnostics, filled by a library built with -DCCS_SEARCH_STATS
+     * (libcubesolver_diag.so; zero otherwise), the counterpart of the reference's
+     * LOOKUP_STATS / LANE_STATS / PHASE_STATS builds (kernels.cu:1485-1505, 1773-1800):
+     *  [0..5]   probes per frame (0..2 forward, 3..5 inverse)
+     *  [6..11]  probes that pruned their child, per frame
+     *  [12]     steps (summed over waves)   [13] frame updates (lanes)
+     *  [14..22] steps by the number of lanes that probe: 0, 1-8, 9-16, ..., 57-64
+     *  [23..26] shader-clock cycles per wave in a step's parts: the pass (bookkeeping,
+     *           fetches, picks), the frame updates, the probes' addresses, the wait
+     *           for the table reads */
+    int32_t diag_built;
+    uint64_t diag[32];
 } ccs_solve_stats_t;
 
 /* Solve n cubes optimally; every returned sequence is verified to solve its
