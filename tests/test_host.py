@@ -105,6 +105,21 @@ def test_kind_info():
         _lib.kind_info("nope")
 
 
+def test_recorded_table_hashes_come_from_complete_builds(table_hashes, reference_results, index_vectors):
+    """Every kind's recorded SHA-256 comes from an oracle BFS whose depth
+    counts equal the reference's (research/subgroups/results.jsonl), and its
+    file sizes are the presets' (kinds.py:115-135)."""
+    sizes = {(p["kind"], p["format"]): p["table_bytes"] for p in index_vectors["presets"]}
+    assert set(table_hashes) == set(reference_results["depth_counts"])
+    for kind, want in table_hashes.items():
+        ref = reference_results["depth_counts"][kind]["depth_counts"]
+        assert want["depth_counts"] == ref, kind
+        assert sum(want["depth_counts"]) == want["entries"] == 9930 * _lib.kind_info(kind).n_edge
+        for fmt, key in (("block", "blk"), ("b3", "b3")):
+            if (kind, fmt) in sizes:
+                assert want[f"{key}_bytes"] == sizes[(kind, fmt)], (kind, fmt)
+
+
 def test_presets_match_reference(index_vectors):
     for want in index_vectors["presets"]:
         p = kinds.PRESET_BY_NAME[want["name"]]
