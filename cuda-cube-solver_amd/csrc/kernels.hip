@@ -1037,6 +1037,13 @@ __device__ __attribute__((noinline)) SplitTake take_split(QueueCtl* q, const Spl
     return r;
 }
 
+#ifdef CCS_SEARCH_STATS
+// Shader clock (s_memtime; its read waits for the wave's outstanding LDS and
+// scalar loads), low 32 bits: differences are taken modulo 2^32.
+constexpr uint32_t CYCLES_MASK = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t shader_cycles() { return (uint32_t)clock64(); }
+#endif
+
 __device__ __forceinline__ Frame load_frame_c(const Frame* p) {
     const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
     return Frame{ld_c(&w[0]), ld_c(&w[1]), ld_c(&w[2]), ld_c(&w[3]), ld_c(&w[4])};
@@ -1104,13 +1111,15 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // a histogram of the lanes that probe in a step, and the shader-clock cycles
     // of a step's parts.
     unsigned long long d_look[6] = {}, d_prune[6] = {}, d_upd = 0;
-    unsigned long long d_steps = 0, d_hist[9] = {}, d_cyc[4] = {};
-    long long d_t0 = 0, d_t1 = 0;
+    unsigned long long d_steps = 0, d_hist[9] = {}, d_cyc[4] = {}, d_istart_cyc = 0, d_istart_steps = 0;
+    uint32_t d_t0 = 0, d_t1 = 0;
+    bool d_start = false;  // the lane started an item in this step's pass
 #endif
 
     for (;;) {
 #ifdef CCS_SEARCH_STATS
-        d_t0 = clock64();
+        d_t0 = shader_cycles();
+        d_start = false;
 #endif
         if (QUEUE) {
             // Donation: every SPLIT_CHECK passes the wave looks at whether lanes are
@@ -1373,6 +1382,9 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         }
                     }
                 }
+#ifdef CCS_SEARCH_STATS
+                d_start = true;
+#endif
                 // ---- start the item: its prefix is replayed move by move through
                 // the probe step below, all six frames (kernels.cu:1552-1614) ----
                 unsigned long long nb;
@@ -1523,8 +1535,12 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             continue;
         }
 #ifdef CCS_SEARCH_STATS
-        d_t1 = clock64();
-        d_cyc[0] += (unsigned long long)(d_t1 - d_t0);  // the pass: bookkeeping, fetches, picks
+        d_t1 = shader_cycles();
+        d_cyc[0] += (d_t1 - d_t0) & CYCLES_MASK;  // the pass: bookkeeping, fetches, picks
+        if (__any(d_start)) {  // passes in which a lane of the wave started an item
+            d_istart_cyc += (d_t1 - d_t0) & CYCLES_MASK;
+            ++d_istart_steps;
+        }
         d_t0 = d_t1;
         ++d_steps;
         ++d_hist[((uint32_t)__popcll(__ballot(eval)) + 7u) >> 3];
@@ -1535,8 +1551,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             if (leaf && is_solved(f.f0)) report_solution(A, J, QUEUE, g, stw, nt);
         }
 #ifdef CCS_SEARCH_STATS
-        d_t1 = clock64();
-        d_cyc[1] += (unsigned long long)(d_t1 - d_t0);  // the frame updates
+        d_t1 = shader_cycles();
+        d_cyc[1] += (d_t1 - d_t0) & CYCLES_MASK;  // the frame updates
         d_t0 = d_t1;
 #endif
         if (eval) {
@@ -1566,12 +1582,19 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             const uint8_t* bI = probe_block<K, F>(L, pkI, table, ti, jI);
             __builtin_amdgcn_s_setprio(0);
 #ifdef CCS_SEARCH_STATS
-            d_t1 = clock64();
-            d_cyc[2] += (unsigned long long)(d_t1 - d_t0);  // the probes' addresses
+            d_t1 = shader_cycles();
+            d_cyc[2] += (d_t1 - d_t0) & CYCLES_MASK;  // the probes' addresses
             d_t0 = d_t1;
 #endif
-            uint32_t vf = hasF ? block_value<F>(L, bF, jF) : 0u;
-            const uint32_t vi = hasI ? block_value<F>(L, bI, jI) : 0u;
+            // Both table reads are issued with no branch around them, so that they
+            // are in flight together: a lane without a probe of one kind reads
+            // the table's first block instead (one L2 line shared by the wave).
+            // Reads guarded by hasF / hasI would each sit in a branch of their
+            // own and wait for their data there, one after the other.
+            const uint32_t tF = block_value<F>(L, hasF ? bF : table, hasF ? jF : 0u);
+            const uint32_t tI = block_value<F>(L, hasI ? bI : table, hasI ? jI : 0u);
+            uint32_t vf = hasF ? tF : 0u;
+            const uint32_t vi = hasI ? tI : 0u;
             probes += (uint32_t)hasF + (uint32_t)hasI;
             const uint32_t pf = hget(hp, (int)kf);
             if (pf) vf = max(vf, pf - 1u);  // forward frames: at least the parent's value - 1
@@ -1593,8 +1616,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 d_prune[j] += hasF && kf == (uint32_t)j && (uint32_t)g + 1u + vf > bound;
                 d_prune[3 + j] += hasI && ki == (uint32_t)j && (uint32_t)g + 1u + vi > bound;
             });
-            d_t1 = clock64();
-            d_cyc[3] += (unsigned long long)(d_t1 - d_t0);  // waiting for the table reads, and the values
+            d_t1 = shader_cycles();
+            d_cyc[3] += (d_t1 - d_t0) & CYCLES_MASK;  // waiting for the table reads, and the values
 #endif
         }
     }
@@ -1618,6 +1641,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             atomicAdd(&A.stats[STAT_DIAG + 13], u);
             for (int i = 0; i < 9; ++i) atomicAdd(&A.stats[STAT_DIAG + 14 + i], d_hist[i]);
             for (int i = 0; i < 4; ++i) atomicAdd(&A.stats[STAT_DIAG + 23 + i], d_cyc[i]);
+            atomicAdd(&A.stats[STAT_DIAG + 27], d_istart_cyc);
+            atomicAdd(&A.stats[STAT_DIAG + 28], d_istart_steps);
         }
 #endif
     }

@@ -166,7 +166,9 @@ typedef struct {
      *  [14..22] steps by the number of lanes that probe: 0, 1-8, 9-16, ..., 57-64
      *  [23..26] shader-clock cycles per wave in a step's parts: the pass (bookkeeping,
      *           fetches, picks), the frame updates, the probes' addresses, the wait
-     *           for the table reads */
+     *           for the table reads
+     *  [27]     pass cycles of the passes in which a lane of the wave started an item
+     *  [28]     steps in which a lane of the wave started an item */
     int32_t diag_built;
     uint64_t diag[32];
 } ccs_solve_stats_t;
