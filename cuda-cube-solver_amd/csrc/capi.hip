@@ -39,6 +39,73 @@ static int class_lut_check() {
     return rc;
 }
 
+// The search keeps a cube's state and its inverse only, and reaches axis
+// frame k through the LUT's corner conjugation records and composite edge
+// tables: for sampled cubes, every frame's table index that way must equal the
+// index of the frame built explicitly (axis_frames).
+template <int K>
+static bool frame_index_check_k(const CubeLUT& L) {
+    const ClassLUT& C = corner_classes().class_lut;
+    uint32_t rng = 12345u;
+    for (int n = 0; n < 256; ++n) {
+        Cubie c = solved();
+        for (int d = 0; d < 25; ++d) {
+            rng = rng * 1664525u + 1013904223u;
+            c = then(c, move((int)((rng >> 8) % 18u)));
+        }
+        Cubie fr[6];
+        axis_frames(c, fr);
+        for (int j = 0; j < 6; ++j) {
+            const Packed want = pack(fr[j]), base = pack(j < 3 ? fr[0] : fr[3]);
+            const Frame fw{want.w[0], want.w[1], want.w[2], want.w[3], want.w[4]};
+            Frame fb{base.w[0], base.w[1], base.w[2], base.w[3], base.w[4]};
+            conj_corners(fb, L.cj[j % 3]);
+            const uint32_t pk = corner_class(C, fb);
+            const uint64_t got = (uint64_t)(pk >> 4) * KindT<K>::N_EDGE + edge_coord<K>(L, fb, (pk & 15u) + 16u * (j % 3));
+            if (got != table_index_c<K>(L, C, fw)) return false;
+        }
+    }
+    return true;
+}
+static bool frame_index_check(const Kind& k, const CubeLUT& L) {
+    switch (kind_K(k)) {
+        case 0: return frame_index_check_k<0>(L);
+        case 8: return frame_index_check_k<8>(L);
+        case 12: return frame_index_check_k<12>(L);
+        case 16: return frame_index_check_k<16>(L);
+        case 24: return frame_index_check_k<24>(L);
+        default: return false;
+    }
+}
+
+int search_format(ccs_tables* t, bool make) {
+    if (t->format != CCS_FORMAT_BLOCK) return t->format;
+    if (t->halves_state == 0 && make) {
+        t->halves_state = -1;
+        const uint64_t nb = t->table_bytes[CCS_FORMAT_BLOCK] / 32;
+        unsigned int* d_bad = nullptr;
+        unsigned int bad = 1;
+        if (hipMalloc(&t->d_halves, nb * 32) == hipSuccess && hipMalloc(&d_bad, 4) == hipSuccess &&
+            hipMemsetAsync(d_bad, 0, 4, t->stream) == hipSuccess &&
+            launch_halves(device_cus() * 16, t->d_table[CCS_FORMAT_BLOCK], t->d_halves, nb, d_bad, t->stream) ==
+                hipSuccess &&
+            hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, t->stream) == hipSuccess &&
+            hipStreamSynchronize(t->stream) == hipSuccess && bad == 0)
+            t->halves_state = 1;
+        (void)hipFree(d_bad);
+        if (t->halves_state != 1) {
+            (void)hipFree(t->d_halves);
+            t->d_halves = nullptr;
+            (void)hipGetLastError();  // an allocation that failed leaves nothing behind
+        }
+    }
+    return t->halves_state == 1 ? FMT_HALVES : CCS_FORMAT_BLOCK;
+}
+
+const uint8_t* search_table(const ccs_tables* t, int F) {
+    return F == FMT_HALVES ? t->d_halves : t->d_table[t->format];
+}
+
 int tables_setup(ccs_tables* t, const std::string& name) {
     std::string err;
     if (!class_lut_check()) return fail(CCS_EFAULT, "corner class tables disagree with classidx");
@@ -47,6 +114,7 @@ int tables_setup(ccs_tables* t, const std::string& name) {
     if (t->K != 0 && t->K != 8 && t->K != 12 && t->K != 16 && t->K != 24)
         return fail(CCS_EINVAL, name + ": no kernel instance for " + std::to_string(t->K) + " E-content classes");
     build_lut(t->kind, t->lut);
+    if (!frame_index_check(t->kind, t->lut)) return fail(CCS_EFAULT, "axis-frame conjugation tables give wrong indices");
     const CornerClasses& cc = corner_classes();
     HIP_TRY(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc(&t->d_lut, sizeof(CubeLUT)));
@@ -457,6 +525,7 @@ int ccs_index_host(const char* kind, const uint8_t* cubes, int n, uint64_t* idx_
     if (!class_lut_check()) return fail(CCS_EFAULT, "corner class tables disagree with classidx");
     std::unique_ptr<CubeLUT> L(new CubeLUT());
     build_lut(k, *L);
+    if (!frame_index_check(k, *L)) return fail(CCS_EFAULT, "axis-frame conjugation tables give wrong indices");
     const ClassLUT& C = corner_classes().class_lut;  // the search's path to the corner class
     for (int i = 0; i < n; ++i) {
         Packed p = pack(*reinterpret_cast<const Cubie*>(cubes + 40 * (size_t)i));

@@ -96,6 +96,12 @@ struct ccs_tables {
     ccs::ClassLUT* d_class_lut = nullptr;  // corner classes from small tables, for the search's LDS
     uint8_t* d_table[2] = {nullptr, nullptr};
     uint64_t table_bytes[2] = {0, 0};
+    // The block table in the search's half-block layout (FMT_HALVES): made at the
+    // first solve or server start on a block table; halves_state 1 made, -1 not
+    // possible (no memory, or a block minimum above 15: searches then read the
+    // block table itself), 0 not tried yet.
+    uint8_t* d_halves = nullptr;
+    int halves_state = 0;
     uint32_t* d_depths = nullptr;  // the BFS build table, kept only on request (CCS_BUILD_KEEP_DEPTHS)
     uint64_t depth_bytes = 0;
     unsigned long long* d_prefix[ccs::MAX_PLEN + 1] = {};  // packed prefixes, 8 B per item
@@ -110,6 +116,7 @@ struct ccs_tables {
         hipFree(d_class_stab);
         hipFree(d_class_lut);
         for (auto* p : d_table) hipFree(p);
+        hipFree(d_halves);
         hipFree(d_depths);
         for (auto* p : d_prefix) hipFree(p);
         if (stream) hipStreamDestroy(stream);
@@ -118,5 +125,9 @@ struct ccs_tables {
 
 namespace ccs {
 int tables_setup(ccs_tables* t, const std::string& name);
+// Table format of the searches (FMT_HALVES for a block table whose half-block
+// copy exists; made here first if `make` and not tried yet), and that table.
+int search_format(ccs_tables* t, bool make);
+const uint8_t* search_table(const ccs_tables* t, int F);
 int ensure_prefixes(ccs_tables* t, int plen);
 }  // namespace ccs

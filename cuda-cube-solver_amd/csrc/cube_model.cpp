@@ -594,6 +594,23 @@ std::vector<uint8_t> canonical_prefixes(int length) {
     return out;
 }
 
+// M_a M_b g M_b^-1 M_a^-1 as one action: conjugation by a after conjugation by b.
+SymAction compose_sym(const SymAction& a, const SymAction& b) {
+    SymAction t{};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            t.mat[3 * r + c] = a.mat[3 * r] * b.mat[c] + a.mat[3 * r + 1] * b.mat[3 + c] + a.mat[3 * r + 2] * b.mat[6 + c];
+    for (int i = 0; i < 8; ++i) {
+        t.cpos[i] = a.cpos[b.cpos[i]];
+        for (int o = 0; o < 3; ++o) t.csig[i][o] = a.csig[b.cpos[i]][b.csig[i][o]];
+    }
+    for (int i = 0; i < 12; ++i) {
+        t.epos[i] = a.epos[b.epos[i]];
+        for (int o = 0; o < 2; ++o) t.esig[i][o] = a.esig[b.epos[i]][b.esig[i][o]];
+    }
+    return t;
+}
+
 void build_lut(const Kind& k, CubeLUT& L) {
     Statics& S = st();
     memset(&L, 0, sizeof L);
@@ -630,8 +647,22 @@ void build_lut(const Kind& k, CubeLUT& L) {
             left_rec(L.lr[1][j][m], fm[j]);
         }
     }
-    for (int s = 0; s < 16; ++s) {
-        const SymAction& y = S.ud[s];
+    // R^k, k = 0..2, and the corner part of conjugation by it (lut.hpp ConjRec)
+    static const int ID[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const SymAction rk[3] = {sym_from_matrix(ID), S.rot, compose_sym(S.rot, S.rot)};
+    for (int kk = 0; kk < 3; ++kk) {
+        const SymAction& y = rk[kk];
+        ConjRec& r = L.cj[kk];
+        for (int i = 0; i < 8; ++i) {
+            const int q = y.cpos[i];
+            put(r.csel[q >> 2], q & 3, (uint32_t)i);
+            put(r.cadd[q >> 2], q & 3, (uint32_t)y.csig[i][0] << 3);
+            put(r.lc[i >> 2], i & 3, (uint32_t)y.cpos[i] | (uint32_t)((3 - y.csig[i][0]) % 3) << 3);
+        }
+    }
+    for (int t = 0; t < N_ESYM; ++t) {
+        const SymAction y = compose_sym(S.ud[t % 16], rk[t / 16]);
+        const int s = t;
         uint8_t pinv[12];
         for (int i = 0; i < 12; ++i) pinv[y.epos[i]] = (uint8_t)i;
         for (int w = 0; w < 3; ++w)

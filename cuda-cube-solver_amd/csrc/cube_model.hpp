@@ -37,6 +37,7 @@ struct SymAction {
 };
 SymAction sym_from_matrix(const int m[9]);
 Cubie conjugate(const SymAction& s, const Cubie& c);  // M g M^-1
+SymAction compose_sym(const SymAction& a, const SymAction& b);  // conjugation by a after b
 
 // The 16 UD-preserving symmetries in the reference's order
 // (geometry.py:96-116) and the URF-diagonal rotation (geometry.py:119-124).

@@ -112,6 +112,15 @@ CCS_HD void left_mul(Frame& f, const LeftRec& r) {
     f = Frame{a, b, e0, e1, e2};
 }
 
+// Corners of R^k g R^-k (lut.hpp ConjRec); the edges are left as they are; the
+// edge coordinate conjugates them through the composite tables instead.
+CCS_HD void conj_corners(Frame& f, const ConjRec& r) {
+    const uint32_t a = twist_mod3(vperm(f.c1, f.c0, r.csel[0]) + r.cadd[0]);
+    const uint32_t b = twist_mod3(vperm(f.c1, f.c0, r.csel[1]) + r.cadd[1]);
+    f.c0 = twist_mod3(vperm(r.lc[1], r.lc[0], a & 0x07070707u) + (a & 0x18181818u));
+    f.c1 = twist_mod3(vperm(r.lc[1], r.lc[0], b & 0x07070707u) + (b & 0x18181818u));
+}
+
 CCS_HD void apply_move(const CubeLUT& L, Frame& f, int m) { right_mul(f, L.rr[0][0][m]); }
 CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) { left_mul(f, L.lr[1][0][m]); }
 
@@ -170,7 +179,8 @@ struct KindT {
 };
 
 // Edge coordinate of M_s g M_s^-1 (coords.py:121-154 for 304; kernels.cu:175-186
-// for the group kinds), from the packed edges.
+// for the group kinds), from the packed edges; s indexes the edge conjugation
+// tables (below 16: a UD symmetry; 16 k + s: that symmetry after R^k).
 template <int K>
 CCS_HD uint32_t edge_coord(const CubeLUT& L, const Frame& f, uint32_t s) {
     uint32_t e[3];
@@ -250,21 +260,43 @@ CCS_HD uint32_t table_lb(const uint8_t* __restrict__ tbl, uint64_t idx) {
 // bracket stays below 2^23 (class * N_EDGE itself reaches 2^35).  Split in two
 // so that a caller can put several probes' table reads in flight before it
 // uses any: probe_block (from the frame's classidx word) and block_value.
+// `frame` k: f holds R^k g R^-k's corners and g's edges (the edges are
+// conjugated by R^k with the class symmetry, in one table).
+//
+// F = 2 (FMT_HALVES) is the block format laid out for the search: each 32-byte
+// block as two self-contained 16-byte halves, entries 0..61 and 62..123, each
+// half the block minimum in its low 4 bits and the entries' 2-bit offsets from
+// bit 4 on.  A probe then reads one 16-byte half with a single load; the
+// values are the block format's, bit for bit.  probe_block returns the half
+// and the entry within it.
+constexpr int FMT_HALVES = 2;
+constexpr uint32_t HALF_ENTRIES = 62;
 template <int K, int F>
-CCS_HD const uint8_t* probe_block(const CubeLUT& L, uint32_t packed, const uint8_t* __restrict__ tbl,
+CCS_HD const uint8_t* probe_block(const CubeLUT& L, uint32_t packed, uint32_t frame, const uint8_t* __restrict__ tbl,
                                   const Frame& f, uint32_t& j) {
-    constexpr uint32_t D = F == 0 ? 124u : 155u;
+    constexpr uint32_t D = F == 1 ? 155u : 124u;
     constexpr uint32_t Q = (uint32_t)(KindT<K>::N_EDGE / D), R = (uint32_t)(KindT<K>::N_EDGE % D);
     static_assert(Q < (1u << 24), "class * Q must stay a 24-bit product");
     const uint32_t cls = packed >> 4;
-    const uint32_t x = mul24(cls, R) + edge_coord<K>(L, f, packed & 15u);
+    const uint32_t x = mul24(cls, R) + edge_coord<K>(L, f, (packed & 15u) + 16u * frame);
     const uint32_t q = x / D;
     j = x - mul24(q, D);
-    return tbl + (uint64_t)(mul24(cls, Q) + q) * 32u;
+    const uint8_t* p = tbl + (uint64_t)(mul24(cls, Q) + q) * 32u;
+    if constexpr (F == FMT_HALVES) {
+        const uint32_t h = j >= HALF_ENTRIES ? 1u : 0u;
+        j -= h * HALF_ENTRIES;
+        p += 16u * h;
+    }
+    return p;
 }
 template <int F>
 CCS_HD uint32_t block_value(const CubeLUT& L, const uint8_t* p, uint32_t j) {
-    if constexpr (F == 0) {
+    if constexpr (F == FMT_HALVES) {
+        const uint4 w = *reinterpret_cast<const uint4*>(p);
+        const uint32_t pos = 4u + 2u * j, k = pos >> 5;
+        const uint32_t word = k == 0u ? w.x : k == 1u ? w.y : k == 2u ? w.z : w.w;
+        return (w.x & 15u) + ((word >> (pos & 31u)) & 3u);
+    } else if constexpr (F == 0) {
         const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
         const uint32_t pos = 8u + 2u * j;
         return (w[0] & 0xFFu) + ((w[pos >> 5] >> (pos & 31u)) & 3u);
@@ -279,7 +311,7 @@ template <int K, int F>
 CCS_HD uint32_t frame_bound(const CubeLUT& L, const uint32_t* __restrict__ classidx,
                             const uint8_t* __restrict__ tbl, const Frame& f) {
     uint32_t j;
-    const uint8_t* p = probe_block<K, F>(L, classidx[corner_raw(L, f)], tbl, f, j);
+    const uint8_t* p = probe_block<K, F>(L, classidx[corner_raw(L, f)], 0u, tbl, f, j);
     return block_value<F>(L, p, j);
 }
 

@@ -532,20 +532,6 @@ __device__ __forceinline__ uint32_t move_axis(int m) {
     uint32_t face = ((uint32_t)m * 11u) >> 5;
     return face >= 3u ? face - 3u : face;
 }
-// The six frames of a lane as named registers (an array indexed in loops can
-// be left in scratch memory by the compiler).
-struct Frames6 {
-    Frame f0, f1, f2, f3, f4, f5;
-    template <int J>
-    __device__ __forceinline__ Frame& at() {
-        if constexpr (J == 0) return f0;
-        else if constexpr (J == 1) return f1;
-        else if constexpr (J == 2) return f2;
-        else if constexpr (J == 3) return f3;
-        else if constexpr (J == 4) return f4;
-        else return f5;
-    }
-};
 template <int I, int N, typename Fn>
 __device__ __forceinline__ void static_for(Fn&& fn) {
     if constexpr (I < N) {
@@ -553,23 +539,10 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
         static_for<I + 1, N>(fn);
     }
 }
-// Frame k of three, selected word by word (whole-struct selects become memory copies).
-__device__ __forceinline__ Frame pick3(const Frame& a, const Frame& b, const Frame& c, uint32_t k) {
-    return Frame{k == 0u ? a.c0 : (k == 1u ? b.c0 : c.c0), k == 0u ? a.c1 : (k == 1u ? b.c1 : c.c1),
-                 k == 0u ? a.e0 : (k == 1u ? b.e0 : c.e0), k == 0u ? a.e1 : (k == 1u ? b.e1 : c.e1),
-                 k == 0u ? a.e2 : (k == 1u ? b.e2 : c.e2)};
-}
 __device__ __forceinline__ Frame load_frame(const Frame* p) {
     const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
     return Frame{w[0], w[1], w[2], w[3], w[4]};
 }
-// Frame j after move m (sel 0) or after undoing move m (sel 1): one code path
-// for both, so that lanes pushing and lanes popping share its instructions.
-__device__ __forceinline__ void frame_move(const CubeLUT& L, Frame& f, int j, uint32_t sel, int m) {
-    if (j < 3) right_mul(f, L.rr[sel][j][m]);
-    else left_mul(f, L.lr[sel][j - 3][m]);
-}
-
 // Ticket of the persistent form: one 64-bit word per cube packs the round
 // (bound, prefix length), the items handed out and the items in flight, so a
 // single atomic both takes items and keeps the round open.
@@ -1070,7 +1043,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     load_lut(&L, A.lut);  // ends with a workgroup barrier
     const uint8_t* __restrict__ table = A.table;
 
-    Frames6 f;
+    // The node's state (axis frame 0) and its inverse (frame 3).  Frame k and
+    // 3 + k are R^k g R^-k of these: a probe conjugates the child's corners
+    // (ConjRec) and the edge coordinate conjugates its edges by R^k with the
+    // class symmetry in one table, so a push or pop moves two frames, not six.
+    Frame f0, f3;
     uint32_t hp = 0, cp = 0;
     uint32_t c = 0xFFFFFFFFu;  // cube of the current item
     SearchJob* J = nullptr;
@@ -1450,11 +1427,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 if (fresh) {
                     hp = 0;
                     if (SERVE) {  // the slot may have held another cube: read past the L2
-                        static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame_c(&J->frame[j]); });
+                        f0 = load_frame_c(&J->frame[0]);
+                        f3 = load_frame_c(&J->frame[3]);
 #pragma unroll
                         for (int j = 0; j < 6; ++j) hp |= min(ld_c(&J->h[j]), 15u) << (4 * j);
                     } else {
-                        static_for<0, 6>([&](auto j) { f.template at<j>() = load_frame(&J->frame[j]); });
+                        f0 = load_frame(&J->frame[0]);
+                        f3 = load_frame(&J->frame[3]);
 #pragma unroll
                         for (int j = 0; j < 6; ++j) hp |= min(J->h[j], 15u) << (4 * j);
                     }
@@ -1547,8 +1526,9 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         d_upd += mv >= 0;
 #endif
         if (mv >= 0) {
-            static_for<0, 6>([&](auto j) { frame_move(L, f.template at<j>(), j, msel, mv); });
-            if (leaf && is_solved(f.f0)) report_solution(A, J, QUEUE, g, stw, nt);
+            right_mul(f0, L.rr[msel][0][mv]);
+            left_mul(f3, L.lr[msel][0][mv]);
+            if (leaf && is_solved(f0)) report_solution(A, J, QUEUE, g, stw, nt);
         }
 #ifdef CCS_SEARCH_STATS
         d_t1 = shader_cycles();
@@ -1573,13 +1553,15 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             const uint32_t kf = ctz(remF | 0x10000u) & 3u, ki = ctz(remI | 0x10000u) & 3u;
             if (hasF) remF &= remF - 1u;
             if (hasI) remI &= remI - 1u;
-            Frame tf = pick3(f.f0, f.f1, f.f2, kf), ti = pick3(f.f3, f.f4, f.f5, ki);
-            right_mul(tf, L.rr[0][kf][m]);
-            left_mul(ti, L.lr[0][ki][m]);
+            Frame tf = f0, ti = f3;
+            right_mul(tf, L.rr[0][0][m]);
+            left_mul(ti, L.lr[0][0][m]);
+            conj_corners(tf, L.cj[kf]);
+            conj_corners(ti, L.cj[ki]);
             const uint32_t pkF = corner_class(CL, tf), pkI = corner_class(CL, ti);
             uint32_t jF, jI;
-            const uint8_t* bF = probe_block<K, F>(L, pkF, table, tf, jF);
-            const uint8_t* bI = probe_block<K, F>(L, pkI, table, ti, jI);
+            const uint8_t* bF = probe_block<K, F>(L, pkF, kf, table, tf, jF);
+            const uint8_t* bI = probe_block<K, F>(L, pkI, ki, table, ti, jI);
             __builtin_amdgcn_s_setprio(0);
 #ifdef CCS_SEARCH_STATS
             d_t1 = shader_cycles();
@@ -1690,6 +1672,23 @@ __global__ void __launch_bounds__(256) gather_kernel(const uint32_t* __restrict_
     atomicAdd(sum, (unsigned long long)acc);
 }
 
+// The block format laid out for the search (FMT_HALVES, device_cube.hpp): half
+// 0 of a block is its bits 4..131, half 1 its bits 128..255, the low 4 bits of
+// each replaced by the block minimum.  Blocks whose minimum does not fit in 4
+// bits are counted in *bad (the layout cannot hold them).
+__global__ void __launch_bounds__(256) halves_kernel(const uint4* __restrict__ blk, uint4* __restrict__ out,
+                                                      unsigned long long n_blocks, unsigned int* bad) {
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long b = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; b < n_blocks; b += stride) {
+        const uint4 lo = blk[2 * b], hi = blk[2 * b + 1];
+        const uint32_t m = lo.x & 0xFFu;
+        if (m > 15u) atomicAdd(bad, 1u);
+        out[2 * b] = make_uint4((((lo.x >> 4) | (lo.y << 28)) & ~15u) | (m & 15u), (lo.y >> 4) | (lo.z << 28),
+                                (lo.z >> 4) | (lo.w << 28), (lo.w >> 4) | (hi.x << 28));
+        out[2 * b + 1] = make_uint4((hi.x & ~15u) | (m & 15u), hi.y, hi.z, hi.w);
+    }
+}
+
 template <int K, int F>
 void* search_fn_f(int form) {
     return form == 2 ? (void*)search_kernel<K, F, true, true>
@@ -1697,7 +1696,7 @@ void* search_fn_f(int form) {
 }
 template <int K>
 void* search_fn(int F, int form) {
-    return F == 0 ? search_fn_f<K, 0>(form) : search_fn_f<K, 1>(form);
+    return F == FMT_HALVES ? search_fn_f<K, FMT_HALVES>(form) : F == 0 ? search_fn_f<K, 0>(form) : search_fn_f<K, 1>(form);
 }
 
 #define CCS_KIND_SWITCH(K, CALL)                  \
@@ -1784,6 +1783,13 @@ hipError_t launch_bfs_layer(int K, bool forward, int grid, const CubeLUT* lut, u
 hipError_t launch_histogram(int grid, const uint32_t* tbl, unsigned long long n_words, unsigned long long n_entries,
                             unsigned long long* counts, hipStream_t s) {
     hipLaunchKernelGGL(histogram_kernel, dim3(grid), dim3(256), 0, s, tbl, n_words, n_entries, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_halves(int grid, const uint8_t* blk, uint8_t* out, unsigned long long n_blocks, unsigned int* bad,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(halves_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<const uint4*>(blk),
+                       reinterpret_cast<uint4*>(out), n_blocks, bad);
     return hipGetLastError();
 }
 

@@ -40,6 +40,11 @@ hipError_t launch_histogram(int grid, const uint32_t* tbl, unsigned long long n_
 hipError_t launch_encode(int format, const uint32_t* tbl, uint8_t* mins, unsigned long long n_entries,
                          unsigned long long b0, unsigned long long nb, uint32_t* out, unsigned int* bad, hipStream_t s);
 
+// The block table in the search's half-block layout (FMT_HALVES); *bad counts
+// blocks whose minimum exceeds 15.
+hipError_t launch_halves(int grid, const uint8_t* blk, uint8_t* out, unsigned long long n_blocks, unsigned int* bad,
+                         hipStream_t s);
+
 // mode 0: random probes; 1: random probes with nontemporal loads; 2: stream the table once.
 hipError_t launch_gather(int mode, int grid, const uint32_t* blk, unsigned long long n_blocks,
                          unsigned long long n_probes, unsigned long long* sum, hipStream_t s);

@@ -34,17 +34,34 @@ struct alignas(16) LeftRec {
     uint32_t pad[3];
 };
 
+// Corners of R^k g R^-k, R the URF-diagonal rotation that turns axis frame 0
+// into frame k: the position part (source corner and twist added per target
+// byte, as in RightRec), then the cubie relabel with its twist (as in
+// LeftRec).  A rotation turns every corner's stickers cyclically, so both
+// twist parts are additions mod 3.
+struct alignas(16) ConjRec {
+    uint32_t csel[2];
+    uint32_t cadd[2];
+    uint32_t lc[2];
+    uint32_t pad[2];
+};
+// Edge conjugation tables: symmetry index 16 k + s conjugates by the UD
+// symmetry s after the axis rotation R^k (k = 0: the UD symmetry alone).
+constexpr int N_ESYM = 48;
+
 struct alignas(16) CubeLUT {
     // [0][j][m] applies move m to frame j (forward) or 3 + j (inverse), [1][j][m]
     // undoes it; [0][0][m] is plain right multiplication by m, [1][0][m] plain
     // left multiplication by m.
     RightRec rr[2][3][18];
     LeftRec lr[2][3][18];
-    // edge conjugation by UD symmetry s (M g M^-1): position part, then cubie relabel
-    uint32_t cs_lo[16][3];
-    uint32_t cs_hi[16][3];
-    uint32_t cs_flip[16][3];  // flip of the source position, << 4 per byte
-    uint32_t cs_rel[16][3];   // new byte for each cubie c: perm(c) | flip(c) << 4
+    ConjRec cj[3];  // corner conjugation by R^k (cj[0]: identity)
+    // edge conjugation by symmetry t = 16 k + s (M g M^-1, M = UD symmetry s after
+    // R^k): position part, then cubie relabel
+    uint32_t cs_lo[N_ESYM][3];
+    uint32_t cs_hi[N_ESYM][3];
+    uint32_t cs_flip[N_ESYM][3];  // flip of the source position, << 4 per byte
+    uint32_t cs_rel[N_ESYM][3];   // new byte for each cubie c: perm(c) | flip(c) << 4
     uint32_t csel[16];        // selector compacting the bytes picked by a 4-bit mask to the low end
     uint16_t slice_lo[64];    // 12C4 lexicographic rank = slice_lo[low 6 bits] + slice_hi[|low|][high 6 bits]
     uint8_t slice_hi[5][64];

@@ -219,7 +219,8 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     A.lut = t->d_lut;
     A.class_lut = t->d_class_lut;
     A.classidx = t->d_classidx;
-    A.table = t->d_table[t->format];
+    const int SF = search_format(t, true);  // the search's half-block copy of a block table, made once
+    A.table = search_table(t, SF);
     A.jobs = s->d_jobs;
     A.q = s->d_q;
     A.ring = s->d_ring;
@@ -254,10 +255,10 @@ int ccs_server_start(ccs_tables_t* t, const ccs_server_opts_t* opts, ccs_server_
     A.host_stop = s->host_stop;
     A.stats = s->d_stats;
     int grid = 0, per_cu = o.blocks_per_cu;
-    if (per_cu <= 0) HIP_TRY(search_occupancy(t->K, t->format, 2, SEARCH_WG, &per_cu));
+    if (per_cu <= 0) HIP_TRY(search_occupancy(t->K, SF, 2, SEARCH_WG, &per_cu));
     grid = std::max(per_cu, 1) * device_cus();
     A.n_lanes = (unsigned long long)grid * SEARCH_WG;
-    HIP_TRY(launch_search(t->K, t->format, 2, A, grid, SEARCH_WG, s->stream));
+    HIP_TRY(launch_search(t->K, SF, 2, A, grid, SEARCH_WG, s->stream));
     s->pump = std::thread(pump_loop, s.get());
     *out = s.release();
     return CCS_OK;

@@ -106,7 +106,7 @@ int wall_clock_khz(int* khz) {
 int search_grid(ccs_tables* t, bool queue, int block, int blocks_per_cu, int* grid) {
     int per_cu = blocks_per_cu;
     if (per_cu <= 0) {
-        HIP_TRY(search_occupancy(t->K, t->format, queue ? 1 : 0, block, &per_cu));
+        HIP_TRY(search_occupancy(t->K, search_format(t, false), queue ? 1 : 0, block, &per_cu));
         if (per_cu <= 0) return fail(CCS_EDEVICE, "search kernel does not fit on a CU");
     }
     *grid = per_cu * device_cus();
@@ -242,7 +242,7 @@ int solve_persistent(ccs_tables* t, const ccs_solve_opts_t& o, SearchArgs& A, in
     }
     hipLaunchKernelGGL(read_clock_kernel, dim3(1), dim3(1), 0, t->stream, d_clock);
     if (events) HIP_TRY(hipEventRecord(ev0, t->stream));
-    HIP_TRY(launch_search(t->K, t->format, 1, A, grid, o.block, t->stream));
+    HIP_TRY(launch_search(t->K, search_format(t, false), 1, A, grid, o.block, t->stream));
     ++launches;
     if (events) HIP_TRY(hipEventRecord(ev1, t->stream));
     hipLaunchKernelGGL(read_clock_kernel, dim3(1), dim3(1), 0, t->stream, d_clock + 1);
@@ -323,7 +323,7 @@ void fill_args(ccs_tables* t, const ccs_solve_opts_t& o, int grid, SearchJob* d_
     A.lut = t->d_lut;
     A.class_lut = t->d_class_lut;
     A.classidx = t->d_classidx;
-    A.table = t->d_table[t->format];
+    A.table = search_table(t, search_format(t, false));
     A.jobs = d_jobs;
     A.prefix_len = o.prefix_len;
     A.max_bound = std::min(o.max_bound, STACK_DEPTH);
@@ -388,6 +388,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     double t_start = now_ms();
     if (stats) memset(stats, 0, sizeof *stats);
     if (n == 0) return CCS_OK;
+    search_format(t, true);  // the search's half-block copy of a block table, made once
     for (int i = 0; i < n; ++i) {
         memset(&out[i], 0, sizeof out[i]);
         out[i].length = -1;
@@ -462,7 +463,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
                 A.prefixes = t->d_prefix[plen];
                 A.n_prefix = t->prefix_n[plen];
                 A.counter = d_counter;
-                HIP_TRY(launch_search(t->K, t->format, 0, A, grid, o.block, t->stream));
+                HIP_TRY(launch_search(t->K, search_format(t, false), 0, A, grid, o.block, t->stream));
                 ++launches;
                 off += kv.second.size();
             }
