@@ -293,6 +293,8 @@ def main() -> None:
     ap.add_argument("--prefix-len", type=int, default=4)
     ap.add_argument("--block", type=int, default=0, help="lanes per search workgroup (0: library default)")
     ap.add_argument("--probes-per-step", type=int, default=0, help="table probes per lane per step, 1 or 2 (0: 2)")
+    ap.add_argument("--read-mode", type=int, default=0,
+                    help="a step's two table reads: 1 paired, 2 in turn (0: the library's choice by batch size)")
     ap.add_argument("--chunk-nodes", type=int, default=0, help="persistent form: nodes per fetched chunk (0: library default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=15.0, help="arrival window per offered load")
@@ -321,7 +323,8 @@ def main() -> None:
         log(f"CPU baseline: {args.cpu_cubes} states on {threads} threads ({cpu_model()}), beside the GPU steps")
         cpu = CpuBaseline(tb, cubes[: args.cpu_cubes], threads)
     knobs = {"chunk_nodes": args.chunk_nodes, "blocks_per_cu": args.blocks_per_cu, "prefix_len": args.prefix_len,
-             "deeper_prefix_at": (5e7, 3e9), "block": args.block, "probes_per_step": args.probes_per_step}
+             "deeper_prefix_at": (5e7, 3e9), "block": args.block, "probes_per_step": args.probes_per_step,
+             "read_mode": args.read_mode}
     tuned = None
     if not args.no_tune:
         from cuda_cube_solver_amd import tune

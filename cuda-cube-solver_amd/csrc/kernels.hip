@@ -1568,15 +1568,23 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             d_cyc[2] += (d_t1 - d_t0) & CYCLES_MASK;  // the probes' addresses
             d_t0 = d_t1;
 #endif
-            // Both table reads are issued with no branch around them, so that they
-            // are in flight together: a lane without a probe of one kind reads
-            // the table's first block instead (one L2 line shared by the wave).
-            // Reads guarded by hasF / hasI would each sit in a branch of their
-            // own and wait for their data there, one after the other.
-            const uint32_t tF = block_value<F>(L, hasF ? bF : table, hasF ? jF : 0u);
-            const uint32_t tI = block_value<F>(L, hasI ? bI : table, hasI ? jI : 0u);
-            uint32_t vf = hasF ? tF : 0u;
-            const uint32_t vi = hasI ? tI : 0u;
+            // The step's table reads.  Paired (A.read_mode 0): both issued with no
+            // branch around them, so that they are in flight together; a lane
+            // without a probe of one kind reads its other probe's block again.
+            // In turn (1): each read in a branch of its own that waits for it;
+            // lanes read only what they probe.  Paired reads are faster in large
+            // batches, where most lanes probe twice; in turn, when most steps are
+            // a child's first, one-probe step (lone and hard cubes).
+            uint32_t vf = 0u, vi = 0u;
+            if (A.read_mode == 0) {
+                const uint32_t tF = block_value<F>(L, hasF ? bF : bI, hasF ? jF : jI);
+                const uint32_t tI = block_value<F>(L, hasI ? bI : bF, hasI ? jI : jF);
+                vf = hasF ? tF : 0u;
+                vi = hasI ? tI : 0u;
+            } else {
+                if (hasF) vf = block_value<F>(L, bF, jF);
+                if (hasI) vi = block_value<F>(L, bI, jI);
+            }
             probes += (uint32_t)hasF + (uint32_t)hasI;
             const uint32_t pf = hget(hp, (int)kf);
             if (pf) vf = max(vf, pf - 1u);  // forward frames: at least the parent's value - 1

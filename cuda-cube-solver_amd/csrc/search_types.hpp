@@ -139,6 +139,7 @@ struct SearchArgs {
     unsigned long long deeper0, deeper1;
     unsigned int poll_mask;      // found flag polled every poll_mask + 1 picks
     int one_probe;               // 1: one table probe per step (the top remaining frame); 0: a forward and an inverse
+    int read_mode;               // a step's two table reads: 0 paired (in flight together), 1 one after the other
     unsigned long long idle_limit;  // persistent form: device-clock ticks a lane may wait for work
     unsigned long long backoff_ticks; // persistent form: idle ticks after which a lane polls less often
     unsigned long long chunk_nodes; // persistent form: target nodes per fetched chunk of items

@@ -80,6 +80,11 @@ void fill_opts(const ccs_solve_opts_t* in, ccs_solve_opts_t& o) {
     if (o.deeper_at[0] == 0 && o.deeper_at[1] == 0) o.deeper_at[0] = 5e7, o.deeper_at[1] = 3e9;
 }
 
+// Batches from this size on read a step's two probes paired (read_mode 1): most
+// of their steps probe twice.  Smaller ones, mostly lone and hard cubes whose
+// steps are mostly a child's first, one-probe step, read in turn (2).
+constexpr int PAIRED_READS_MIN_CUBES = 1024;
+
 int check_opts(const ccs_solve_opts_t& o) {
     if (o.mode != 0 && o.mode != 1) return fail(CCS_EINVAL, "mode must be 0 (plain) or 1 (persistent)");
     if (o.prefix_len < 4 || o.prefix_len > MAX_PLEN) return fail(CCS_EINVAL, "prefix_len must be 4..7");
@@ -87,6 +92,7 @@ int check_opts(const ccs_solve_opts_t& o) {
     if (o.block < 64 || o.block > SEARCH_WG || o.block % 64)
         return fail(CCS_EINVAL, "block must be a multiple of 64 up to " + std::to_string(SEARCH_WG));
     if (o.probes_per_step != 1 && o.probes_per_step != 2) return fail(CCS_EINVAL, "probes_per_step must be 1 or 2");
+    if (o.read_mode < 0 || o.read_mode > 2) return fail(CCS_EINVAL, "read_mode must be 0, 1 or 2");
     if (o.poll_interval & (o.poll_interval - 1)) return fail(CCS_EINVAL, "poll_interval must be a power of two");
     return CCS_OK;
 }
@@ -331,6 +337,7 @@ void fill_args(ccs_tables* t, const ccs_solve_opts_t& o, int grid, SearchJob* d_
     A.deeper1 = to_u64(o.deeper_at[1]);
     A.poll_mask = (unsigned)o.poll_interval - 1u;
     A.one_probe = o.probes_per_step == 1;
+    A.read_mode = o.read_mode == 1 ? 0 : 1;
     A.chunk_nodes = o.chunk_nodes > 0 ? (unsigned long long)o.chunk_nodes : DEFAULT_CHUNK_NODES;
     A.n_lanes = (unsigned long long)grid * (unsigned long long)o.block;
     A.stats = d_stats;
@@ -385,6 +392,7 @@ extern "C" int ccs_solve(ccs_tables_t* t, const uint8_t* cubes, int n, const ccs
     fill_opts(opts_in, o);
     int rc = check_opts(o);
     if (rc) return rc;
+    if (o.read_mode == 0) o.read_mode = n >= PAIRED_READS_MIN_CUBES ? 1 : 2;
     double t_start = now_ms();
     if (stats) memset(stats, 0, sizeof *stats);
     if (n == 0) return CCS_OK;
@@ -526,6 +534,7 @@ extern "C" int ccs_solve_noalloc(ccs_tables_t* t, const uint8_t* cubes, int n, c
     o.mode = 1;  // the persistent form only: all rounds in one launch
     int rc = check_opts(o);
     if (rc) return rc;
+    if (o.read_mode == 0) o.read_mode = n >= PAIRED_READS_MIN_CUBES ? 1 : 2;
     for (int p = 0; p <= std::min(o.prefix_len + 2, MAX_PLEN); ++p)
         if (!t->prefix_n[p])
             return fail(CCS_EINVAL, "prefix table " + std::to_string(p) +

@@ -53,7 +53,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("prefix_len", ctypes.c_int32), ("deeper_at", ctypes.c_double * 2),
                 ("block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("max_bound", ctypes.c_int32),
                 ("poll_interval", ctypes.c_int32), ("chunk_nodes", ctypes.c_int32),
-                ("probes_per_step", ctypes.c_int32)]
+                ("probes_per_step", ctypes.c_int32), ("read_mode", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -71,7 +71,7 @@ class SolveStats(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("state_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int32),
                 ("prefix_len", ctypes.c_int32), ("deeper_at", ctypes.c_double * 2),
-                ("blocks_per_cu", ctypes.c_int32), ("chunk_nodes", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32), ("chunk_nodes", ctypes.c_int32), ("read_mode", ctypes.c_int32)]
 
 
 class Served(ctypes.Structure):

@@ -26,7 +26,8 @@ class Server:
     (admission stops above it); ``max_inflight`` caps them further."""
 
     def __init__(self, tables: Tables, state_budget_bytes: int = 0, max_inflight: int = 0,
-                 prefix_len: int = 4, deeper_prefix_at=(0.0, 0.0), blocks_per_cu: int = 0, chunk_nodes: int = 0):
+                 prefix_len: int = 4, deeper_prefix_at=(0.0, 0.0), blocks_per_cu: int = 0, chunk_nodes: int = 0,
+                 read_mode: int = 0):
         o = _lib.ServerOpts()
         o.state_budget_bytes = state_budget_bytes
         o.max_inflight = max_inflight
@@ -34,6 +35,7 @@ class Server:
         o.deeper_at[0], o.deeper_at[1] = deeper_prefix_at
         o.blocks_per_cu = blocks_per_cu
         o.chunk_nodes = chunk_nodes
+        o.read_mode = read_mode  # a step's two table reads: 1 paired, 2 in turn (0: the library's choice)
         self._h = ctypes.c_void_p()
         self.tables = tables
         _lib.check(_lib.lib().ccs_server_start(tables.handle, ctypes.byref(o), ctypes.byref(self._h)))

@@ -167,7 +167,7 @@ class GpuSolver:
     def __init__(self, tables: Path = Path("tables"), kind: str = "d4t", table_format: str = "block",
                  mode: str = "plain", prefix_len: int = 4, deeper_prefix_at=(5e7, 3e9), blocks_per_cu: int = 0,
                  poll_interval: int = 64, chunk_nodes: int = 0, table: Tables | None = None, max_bound: int = 0,
-                 block: int = 0, probes_per_step: int = 0):
+                 block: int = 0, probes_per_step: int = 0, read_mode: int = 0):
         if table is None:
             tables = Path(tables)
             if tables.suffix == ".bin":
@@ -185,6 +185,7 @@ class GpuSolver:
         self.max_bound = max_bound  # 0: no limit below the longest optimal solution (20)
         self.block = block  # lanes per workgroup (0: the largest the kernels are built for)
         self.probes_per_step = probes_per_step  # 1 or 2 table probes in flight per lane (0: 2)
+        self.read_mode = read_mode  # a step's two table reads: 1 paired, 2 in turn (0: by batch size)
         self.last_stats: _lib.SolveStats | None = None
 
     def _opts(self, mode: str | None = None) -> _lib.SolveOpts:
@@ -198,6 +199,7 @@ class GpuSolver:
         o.poll_interval = self.poll_interval
         o.chunk_nodes = self.chunk_nodes
         o.probes_per_step = self.probes_per_step
+        o.read_mode = self.read_mode
         return o
 
     def solve(self, states: list[CubieState], log=None, mode: str | None = None) -> list[Solution]:

@@ -5,7 +5,8 @@ fastest: nodes per fetched chunk (node budget), prefix length, the per-round
 prefix policy (the node counts at which a round's items get one and two moves
 longer), then, at the best of those, waves per CU (lanes per workgroup; the
 LUTs and the lanes' stacks fill the LDS, so one workgroup runs per CU) and
-table probes in flight per lane per step.  Every setting gives the same
+table probes in flight per lane per step, and whether a step's two table reads
+are issued together or in turn.  Every setting gives the same
 optimal lengths; the tuner checks that too.
 
     python -m cuda_cube_solver_amd.tune --table d4t --cubes 2048 --seed 33 [--out knobs.json]
@@ -27,16 +28,18 @@ PREFIX_LEN = (4, 5)
 DEEPER_AT = ((5e7, 3e9), (1e7, 1e9))
 BLOCK = (768, 512)           # lanes per workgroup: 12 or 8 waves per CU
 PROBES_PER_STEP = (2, 1)
+READ_MODE = (1, 2)           # a step's two table reads paired or in turn
 
 
 def tune(tables: Tables, cubes, log=None) -> dict:
     """Best knobs for solving `cubes` on `tables` in the batched form."""
     trials, lengths = [], None
 
-    def run(chunk, bpc, plen, deeper, block=BLOCK[0], probes=PROBES_PER_STEP[0]):
+    def run(chunk, bpc, plen, deeper, block=BLOCK[0], probes=PROBES_PER_STEP[0], reads=READ_MODE[0]):
         nonlocal lengths
         solver = GpuSolver(table=tables, mode="batched", prefix_len=plen, deeper_prefix_at=deeper,
-                           blocks_per_cu=bpc, chunk_nodes=chunk, block=block, probes_per_step=probes)
+                           blocks_per_cu=bpc, chunk_nodes=chunk, block=block, probes_per_step=probes,
+                           read_mode=reads)
         t = time.perf_counter()
         sols = solver.solve(cubes)
         dt = time.perf_counter() - t
@@ -44,9 +47,9 @@ def tune(tables: Tables, cubes, log=None) -> dict:
         if lengths is None:
             lengths = got
         elif got != lengths:
-            raise AssertionError(f"knobs {chunk, bpc, plen, deeper, block, probes} changed the optimal lengths")
+            raise AssertionError(f"knobs {chunk, bpc, plen, deeper, block, probes, reads} changed the optimal lengths")
         rec = {"chunk_nodes": chunk, "blocks_per_cu": bpc, "prefix_len": plen, "deeper_prefix_at": list(deeper),
-               "block": block, "probes_per_step": probes, "cubes_per_s": round(len(cubes) / dt, 2)}
+               "block": block, "probes_per_step": probes, "read_mode": reads, "cubes_per_s": round(len(cubes) / dt, 2)}
         trials.append(rec)
         if log:
             log(f"tune: {rec}")
@@ -63,6 +66,8 @@ def tune(tables: Tables, cubes, log=None) -> dict:
         run(*keep, block=block)
     for probes in PROBES_PER_STEP[1:]:
         run(*keep, probes=probes)
+    for reads in READ_MODE[1:]:
+        run(*keep, reads=reads)
     best = max(trials, key=lambda r: r["cubes_per_s"])
     return {"best": best, "trials": trials, "sample": len(cubes)}
 
@@ -71,7 +76,8 @@ def knobs(best: dict) -> dict:
     """GpuSolver keyword arguments of a tuner result."""
     return {"chunk_nodes": best["chunk_nodes"], "blocks_per_cu": best["blocks_per_cu"],
             "prefix_len": best["prefix_len"], "deeper_prefix_at": tuple(best["deeper_prefix_at"]),
-            "block": best.get("block", 0), "probes_per_step": best.get("probes_per_step", 0)}
+            "block": best.get("block", 0), "probes_per_step": best.get("probes_per_step", 0),
+            "read_mode": best.get("read_mode", 0)}
 
 
 def main(argv=None) -> int:

@@ -139,6 +139,9 @@ typedef struct {
     int32_t probes_per_step; /* table probes a lane keeps in flight per step, 1 or 2 (0: 2): a child's
                                first step probes its top frame alone, later steps one forward and one
                                inverse frame (2) or the top remaining frame (1); nodes are the same */
+    int32_t read_mode;       /* a step's two table reads: 1 paired (issued together; a lane with one
+                               probe reads its block twice), 2 in turn (each read waited for before the
+                               next), 0: paired for batches of at least 1,024 cubes, else in turn */
 } ccs_solve_opts_t;
 
 typedef struct {
@@ -212,6 +215,7 @@ typedef struct {
     int32_t chunk_nodes;         /* target nodes per chunk of items a lane takes (0: 16384); a
                                     lane covers ~40k nodes/s, so this bounds how long one chunk
                                     holds a cube back */
+    int32_t read_mode;           /* as in ccs_solve_opts_t: 1 paired, 2 in turn (0: paired) */
 } ccs_server_opts_t;
 
 typedef struct {

@@ -145,6 +145,20 @@ def test_batched_matches_plain(td4t, reference_results):
     assert [_complete_nodes(s) for s in batched] == [_complete_nodes(s) for s in plain]
 
 
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_read_modes_give_the_reference_nodes(td4t, reference_results, mode):
+    """A step's two table reads paired (1) or in turn (2), and one probe per
+    step: the reference's lengths and completed-round node total on its first
+    100 random positions (reads change the timing, never the values)."""
+    ref = reference_results["random1000"]
+    cubes = _scrambled(ref["scrambles"][:100])
+    want = round(reference_results["complete_round_nodes"]["d4t"]["random_nodes"] * 100)
+    for reads, probes in ((1, 2), (2, 2), (1, 1)):
+        sols = GpuSolver(table=td4t, mode=mode, read_mode=reads, probes_per_step=probes).solve(cubes)
+        assert [len(s.moves) for s in sols] == [r["len"] for r in ref["results"][:100]], (reads, probes)
+        assert sum(_complete_nodes(s) for s in sols) == want, (reads, probes)
+
+
 def test_b3_table_gives_the_same_lengths(td4t, reference_results):
     ref = reference_results["random1000"]
     cubes = _scrambled(ref["scrambles"][200:232])

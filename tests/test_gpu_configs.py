@@ -172,7 +172,7 @@ def test_autotuner_on_a_small_sample(td4t, seeded):
     from cuda_cube_solver_amd import tune
 
     res = tune.tune(td4t, states.uniform_states(48, 33))
-    assert res["sample"] == 48 and len(res["trials"]) == 9
+    assert res["sample"] == 48 and len(res["trials"]) == 10
     best = res["best"]
     assert best in res["trials"] and best["cubes_per_s"] == max(t["cubes_per_s"] for t in res["trials"])
     cubes = states.uniform_states(16, 3)
