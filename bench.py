@@ -36,8 +36,8 @@ BYTES_PER_PROBE = 32  # algorithmic: one 32-byte table block per lookup
 # WRITE_SIZE (KiB per dispatch) scaled by kernels of known size in the same run
 # (FETCH_SIZE x2: it books each 128-byte read request as 64 B; WRITE_SIZE x1):
 # profiles/search_kernel_d4t_pmc.md.  A random 32-byte block costs one 128-byte line.
-FETCHED_BYTES_PER_PROBE = 128.0
-WRITTEN_BYTES_PER_PROBE = 0.059
+FETCHED_BYTES_PER_PROBE = 128.1
+WRITTEN_BYTES_PER_PROBE = 0.061
 
 
 def log(msg: str) -> None:

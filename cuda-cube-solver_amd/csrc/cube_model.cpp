@@ -639,13 +639,10 @@ void build_lut(const Kind& k, CubeLUT& L) {
     };
     auto inverse = [](int m) { return 3 * (m / 3) + 2 - m % 3; };
     for (int m = 0; m < 18; ++m) {
-        const int fm[3] = {m, S.axis_map[m], S.axis_map[S.axis_map[m]]};  // m as axis frame j sees it
-        for (int j = 0; j < 3; ++j) {
-            right_rec(L.rr[0][j][m], fm[j]);
-            right_rec(L.rr[1][j][m], inverse(fm[j]));
-            left_rec(L.lr[0][j][m], inverse(fm[j]));
-            left_rec(L.lr[1][j][m], fm[j]);
-        }
+        right_rec(L.rr[0][m], m);
+        right_rec(L.rr[1][m], inverse(m));
+        left_rec(L.lr[0][m], inverse(m));
+        left_rec(L.lr[1][m], m);
     }
     // R^k, k = 0..2, and the corner part of conjugation by it (lut.hpp ConjRec)
     static const int ID[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};

@@ -121,8 +121,8 @@ CCS_HD void conj_corners(Frame& f, const ConjRec& r) {
     f.c1 = twist_mod3(vperm(r.lc[1], r.lc[0], b & 0x07070707u) + (b & 0x18181818u));
 }
 
-CCS_HD void apply_move(const CubeLUT& L, Frame& f, int m) { right_mul(f, L.rr[0][0][m]); }
-CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) { left_mul(f, L.lr[1][0][m]); }
+CCS_HD void apply_move(const CubeLUT& L, Frame& f, int m) { right_mul(f, L.rr[0][m]); }
+CCS_HD void apply_left(const CubeLUT& L, Frame& f, int m) { left_mul(f, L.lr[1][m]); }
 
 // Byte i (0..15) of a 16-byte table held in four words, by v_perm.
 CCS_HD uint32_t byte16(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t i) {

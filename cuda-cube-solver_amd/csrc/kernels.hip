@@ -1526,8 +1526,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         d_upd += mv >= 0;
 #endif
         if (mv >= 0) {
-            right_mul(f0, L.rr[msel][0][mv]);
-            left_mul(f3, L.lr[msel][0][mv]);
+            right_mul(f0, L.rr[msel][mv]);
+            left_mul(f3, L.lr[msel][mv]);
             if (leaf && is_solved(f0)) report_solution(A, J, QUEUE, g, stw, nt);
         }
 #ifdef CCS_SEARCH_STATS
@@ -1554,8 +1554,8 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             if (hasF) remF &= remF - 1u;
             if (hasI) remI &= remI - 1u;
             Frame tf = f0, ti = f3;
-            right_mul(tf, L.rr[0][0][m]);
-            left_mul(ti, L.lr[0][0][m]);
+            right_mul(tf, L.rr[0][m]);
+            left_mul(ti, L.lr[0][m]);
             conj_corners(tf, L.cj[kf]);
             conj_corners(ti, L.cj[ki]);
             const uint32_t pkF = corner_class(CL, tf), pkI = corner_class(CL, ti);

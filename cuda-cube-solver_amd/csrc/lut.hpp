@@ -17,9 +17,9 @@ namespace ccs {
 
 constexpr uint32_t SEL_ZERO = 0x0Cu;
 
-// Move records.  A forward frame j (the cube seen in axis frame j) takes a
-// move as right multiplication by the move as that frame sees it; the inverse
-// frame 3 + j takes it as left multiplication by that move's inverse.
+// Move records.  The search's state takes a move as right multiplication by
+// it; the state's inverse takes it as left multiplication by the move's
+// inverse.  (Axis frames 1 and 2 are reached by conjugation, ConjRec below.)
 struct alignas(16) RightRec {
     uint32_t csel[2];   // corner word w: source corner index per byte (perm over c1:c0)
     uint32_t cadd[2];   // co << 3 added per byte
@@ -50,11 +50,11 @@ struct alignas(16) ConjRec {
 constexpr int N_ESYM = 48;
 
 struct alignas(16) CubeLUT {
-    // [0][j][m] applies move m to frame j (forward) or 3 + j (inverse), [1][j][m]
-    // undoes it; [0][0][m] is plain right multiplication by m, [1][0][m] plain
-    // left multiplication by m.
-    RightRec rr[2][3][18];
-    LeftRec lr[2][3][18];
+    // rr[0][m] applies move m to the state (right multiplication by m), rr[1][m]
+    // undoes it; lr[0][m] applies it to the inverse (left multiplication by
+    // m^-1), lr[1][m] undoes that (left multiplication by m).
+    RightRec rr[2][18];
+    LeftRec lr[2][18];
     ConjRec cj[3];  // corner conjugation by R^k (cj[0]: identity)
     // edge conjugation by symmetry t = 16 k + s (M g M^-1, M = UD symmetry s after
     // R^k): position part, then cubie relabel
