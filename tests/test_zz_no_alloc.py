@@ -66,3 +66,24 @@ def test_noalloc_limits(td4t):
     rc, got, _ = _noalloc(td4t, cubes)
     _lib.check(rc)
     assert [r.length for r in got] == [len(s.moves) for s in GpuSolver(table=td4t, mode="batched").solve(cubes)]
+
+
+def test_block_layout_search_equals_the_half_block_search(reference_results):
+    """On a fresh handle the allocation-free solve searches the block table
+    itself (it never makes the half-block copy); the first ccs_solve then
+    makes the copy and searches that.  Both give the same lengths and the same
+    completed-round nodes: the half-block layout holds the block format's
+    values bit for bit."""
+    tb, counts = Tables.build("304", "block")
+    try:
+        assert counts == reference_results["depth_counts"]["304"]["depth_counts"]
+        cubes = states.uniform_states(48, 96)
+        rc, got, _ = _noalloc(tb, cubes)  # block layout
+        _lib.check(rc)
+        want = GpuSolver(table=tb, mode="batched").solve(cubes)  # half-block layout
+        assert [r.length for r in got] == [len(s.moves) for s in want]
+        for r, s in zip(got, want):
+            L = r.length
+            assert [x for x in _rounds(r) if x[0] < L] == [x for x in s.iteration_nodes if x[0] < L]
+    finally:
+        tb.close()
