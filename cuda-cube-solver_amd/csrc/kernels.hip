@@ -1072,6 +1072,15 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // read one item ahead so that no item start waits on a global load.
     unsigned long long pnext = 0;
     uint32_t found_seen = 0;
+    // Persistent form: the current chunk's round data, read once when the chunk
+    // is taken (per-item reads of the kernel arguments indexed by the lane's
+    // prefix length, and of the cube's band bounds, each cost a memory round
+    // trip at every item start): the prefix table and its item count, the key
+    // rows the round reads and writes, and its band's key bounds (lo | hi << 8).
+    const unsigned long long* ptab = nullptr;
+    uint32_t pn = 0, band_lh = 0;
+    const unsigned long long* krd = nullptr;
+    uint32_t* kwr = nullptr;
     // persistent form: the lane's chunk: items (visits) left in the low byte,
     // log2 chunk size << 8, CL_BANDED, CL_KEYS
     uint32_t chunk_left = 0, item_cur = 0;
@@ -1089,6 +1098,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
     // of a step's parts.
     unsigned long long d_look[6] = {}, d_prune[6] = {}, d_upd = 0;
     unsigned long long d_steps = 0, d_hist[9] = {}, d_cyc[4] = {}, d_istart_cyc = 0, d_istart_steps = 0;
+    unsigned long long d_iters = 0;  // iterations of the pass loop (the wave's)
     uint32_t d_t0 = 0, d_t1 = 0;
     bool d_start = false;  // the lane started an item in this step's pass
 #endif
@@ -1119,6 +1129,9 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
         uint32_t msel = 0;
         bool leaf = false;
         while (!done) {
+#ifdef CCS_SEARCH_STATS
+            d_iters += __builtin_amdgcn_readfirstlane(1u);
+#endif
             // One pass resolves the lane's evaluated child, then finishes, fetches,
             // undoes or backs up as needed and picks the next child, so that the
             // common sequences (push then pick, discard then pick, pop then pick)
@@ -1150,15 +1163,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                         phase = 3;
                         todo &= ~((1u << (uint32_t)((pbits >> (8 * g)) & 0xFFu)) - 1u);
                         if (QUEUE && !SERVE && (chunk_left & CL_KEYS)) {  // the item's key, for the next round's order
-                            const uint32_t n_items = A.pre_n[plen];
-                            const uint32_t pi = (chunk_left & CL_BANDED) ? item_cur % n_items : item_cur;
+                            const uint32_t pi = (chunk_left & CL_BANDED) ? item_cur % pn : item_cur;
                             uint32_t key = 1u;
 #pragma unroll
                             for (int j = 0; j < 6; ++j) key += hget(hp, j);
                             // this round's buffer: the round reads only the other one, so an
                             // item's key never changes under the round's own visits
-                            atomicOr(&A.keys[plen][((size_t)c * 2u + (bound & 1u)) * A.key_words[plen] + (pi >> 2)],
-                                     key << (8u * (pi & 3u)));
+                            atomicOr(kwr + (pi >> 2), key << (8u * (pi & 3u)));
                             if ((pi & 7u) == 0u) atomicAdd(&A.key_hist[(size_t)c * KEY_BINS + key], 1u);
                         }
                     }
@@ -1220,13 +1231,12 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     plen = (uint32_t)A.plen;
                 } else if (chunk_left & 0xFFu) {
                     // next item of this lane's chunk: same cube and round, no atomics
-                    const uint32_t n_items = A.pre_n[plen];
-                    const uint32_t bbase = (!SERVE && (chunk_left & CL_BANDED)) ? (item_cur / n_items) * n_items : 0u;
-                    const uint32_t w = next_item(item_cur - bbase, chunk_left & 0xFFu, (chunk_left >> 8) & 7u, n_items);
+                    const uint32_t bbase = (!SERVE && (chunk_left & CL_BANDED)) ? (item_cur / pn) * pn : 0u;
+                    const uint32_t w = next_item(item_cur - bbase, chunk_left & 0xFFu, (chunk_left >> 8) & 7u, pn);
                     item = bbase + w;
                     --chunk_left;
                     fresh = false;
-                    if (w >= n_items) {  // the rest of the chunk lies past the round's (band's) last item
+                    if (w >= pn) {  // the rest of the chunk lies past the round's (band's) last item
                         chunk_left = 0;
                         end_chunk(A, c, J, chunk_nodes);
                         continue;
@@ -1321,6 +1331,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             J = &A.jobs[c];
                             bound = st.info & 31u;
                             plen = (st.info >> 8) & 31u;  // the path: replayed like a prefix
+                            found_seen = 0;
                             item = 0;
                             item_cur = 0;
                             fresh = true;
@@ -1338,10 +1349,23 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             J = &A.jobs[c];
                             bound = t_bound(old);
                             plen = pl;
+                            ptab = A.pre_tab[pl];
+                            pn = A.pre_n[pl];
+                            if (!SERVE && A.keys[pl]) {
+                                uint32_t* krow0 = A.keys[pl] + (size_t)c * 2u * A.key_words[pl];
+                                kwr = krow0 + (bound & 1u) * A.key_words[pl];
+                                krd = reinterpret_cast<const unsigned long long*>(krow0 + ((bound - 1u) & 1u) * A.key_words[pl]);
+                            }
                             if (!SERVE && (old & T_BANDED)) {  // chunk ci of the round is chunk ci mod Q of band ci / Q
-                                const uint32_t n_items = A.pre_n[pl], nq = (n_items + (1u << clog) - 1u) >> clog;
+                                const uint32_t nq = (pn + (1u << clog) - 1u) >> clog;
                                 const uint32_t ci = item >> clog, band = ci / nq;
-                                item = band * n_items + (ci - band * nq) * min(ITEM_BLOCK, 1u << clog);
+                                item = band * pn + (ci - band * nq) * min(ITEM_BLOCK, 1u << clog);
+                                // the band's key bounds, fixed for the round (close_round set them)
+                                const uint32_t lo =
+                                    band ? (ld_c(&J->band_thr[(band - 1u) >> 2]) >> (8u * ((band - 1u) & 3u))) & 0xFFu : 0u;
+                                const uint32_t hi =
+                                    band < N_BANDS - 1u ? (ld_c(&J->band_thr[band >> 2]) >> (8u * (band & 3u))) & 0xFFu : 255u;
+                                band_lh = lo | hi << 8;
                             } else {
                                 item = (item >> clog) * min(ITEM_BLOCK, 1u << clog);  // chunk's first item
                             }
@@ -1356,6 +1380,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                                 end_chunk(A, c, J, chunk_nodes);
                                 continue;
                             }
+                            found_seen = 0;  // refreshed by the picks' polls
                         }
                     }
                 }
@@ -1371,20 +1396,15 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     if (banded) {
                         // visit `item` is item pitem of band `band`: searched only there,
                         // by the keys of the previous round (its buffer: bound - 1)
-                        const uint32_t n_items = A.pre_n[plen];
-                        const uint32_t band = item / n_items;
-                        pitem = item - band * n_items;
-                        const unsigned long long* krow = reinterpret_cast<const unsigned long long*>(
-                            A.keys[plen] + ((size_t)c * 2u + ((bound - 1u) & 1u)) * A.key_words[plen]);
-                        const unsigned long long kw = ld_c(&krow[pitem >> 3]);  // keys of 8 items
-                        const uint32_t lo = band ? (ld_c(&J->band_thr[(band - 1u) >> 2]) >> (8u * ((band - 1u) & 3u))) & 0xFFu : 0u;
-                        const uint32_t hi =
-                            band < N_BANDS - 1u ? (ld_c(&J->band_thr[band >> 2]) >> (8u * (band & 3u))) & 0xFFu : 255u;
+                        const uint32_t band = item / pn;
+                        pitem = item - band * pn;
+                        const unsigned long long kw = ld_c(&krd[pitem >> 3]);  // keys of 8 items
+                        const uint32_t lo = band_lh & 0xFFu, hi = band_lh >> 8;
                         bool member = in_band((uint32_t)(kw >> (8u * (pitem & 7u))) & 0xFFu, band, lo, hi);
                         // the chunk's next visits whose keys came with the same load (the
                         // rest of its block of ITEM_BLOCK consecutive items)
                         const uint32_t blk = min(ITEM_BLOCK, 1u << ((chunk_left >> 8) & 7u));
-                        while (!member && (chunk_left & 0xFFu & (blk - 1u)) && ((pitem + 1u) & 7u) && pitem + 1u < n_items) {
+                        while (!member && (chunk_left & 0xFFu & (blk - 1u)) && ((pitem + 1u) & 7u) && pitem + 1u < pn) {
                             ++item;
                             ++pitem;
                             --chunk_left;
@@ -1402,13 +1422,11 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                             break;
                         }
                     }
-                    nb = spl ? spath : (fresh || banded) ? A.pre_tab[plen][pitem] : pnext;
+                    nb = spl ? spath : (fresh || banded) ? ptab[pitem] : pnext;
                     if ((chunk_left & 0xFFu) && !banded) {  // the next item's prefix, used when it starts
-                        const uint32_t n_items = A.pre_n[plen];
-                        const uint32_t nx = next_item(item, chunk_left & 0xFFu, (chunk_left >> 8) & 7u, n_items);
-                        if (nx < n_items) pnext = A.pre_tab[plen][nx];
+                        const uint32_t nx = next_item(item, chunk_left & 0xFFu, (chunk_left >> 8) & 7u, pn);
+                        if (nx < pn) pnext = ptab[nx];
                     }
-                    found_seen = ld_c(&J->found);  // looked at when this item ends
                 } else {
                     nb = A.prefixes[item];
                 }
@@ -1492,9 +1510,10 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                     phase = 4;
                     continue;
                 }
-                // refresh the flag every poll_mask + 1 picks; it is read from the next pick on,
-                // so the load's latency hides behind this step's probes
-                if ((nodes & A.poll_mask) == 0) found_seen = ld_c(&J->found);
+                // refresh the flag every poll_mask + 1 picks of the lane's chunk (its items
+                // start with no read of it); it is read from the next pick on, so the
+                // load's latency hides behind this step's probes
+                if ((((QUEUE ? chunk_nodes : 0ull) + nodes) & A.poll_mask) == 0) found_seen = ld_c(&J->found);
                 // ---- pick the next canonical child ----
                 m = (int)ctz(todo);
                 todo &= todo - 1;
@@ -1633,6 +1652,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
             for (int i = 0; i < 4; ++i) atomicAdd(&A.stats[STAT_DIAG + 23 + i], d_cyc[i]);
             atomicAdd(&A.stats[STAT_DIAG + 27], d_istart_cyc);
             atomicAdd(&A.stats[STAT_DIAG + 28], d_istart_steps);
+            atomicAdd(&A.stats[STAT_DIAG + 29], d_iters);
         }
 #endif
     }

@@ -19,7 +19,7 @@ constexpr unsigned long long SERVE_CHUNK_NODES = 16384;
 constexpr int MAX_PLEN = 7;     // longest work-item prefix
 constexpr int STAT_WORDS = 40;  // device statistics words of a search launch
 constexpr int STAT_DIAG = 8;    // first diagnostics word (DIAG_WORDS of them)
-constexpr int DIAG_WORDS = 29;
+constexpr int DIAG_WORDS = 30;
 // Persistent batches of at least BAND_MIN_CUBES cubes: item keys are kept for
 // rounds with prefixes of at most KEY_MAX_PLEN moves, and a long round (at
 // least BAND_MIN_NODES_PER_ITEM nodes per item in the round before) whose

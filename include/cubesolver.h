@@ -171,7 +171,9 @@ typedef struct {
      *           fetches, picks), the frame updates, the probes' addresses, the wait
      *           for the table reads
      *  [27]     pass cycles of the passes in which a lane of the wave started an item
-     *  [28]     steps in which a lane of the wave started an item */
+     *  [28]     steps in which a lane of the wave started an item
+     *  [29]     iterations of the pass loop (summed over waves; a wave's pass
+     *           runs until each of its lanes has a probe or a frame update) */
     int32_t diag_built;
     uint64_t diag[32];
 } ccs_solve_stats_t;

@@ -44,6 +44,7 @@ def test_diagnostics_build_counts_add_up():
     assert sum(d[14:23]) == d[12] > 0                         # the histogram covers every step
     assert all(c > 0 for c in d[23:27])
     assert 0 < d[27] <= d[23] and 0 < d[28] <= d[12]        # passes with an item start
+    assert d[29] >= d[12]                                    # at least one pass iteration per step
     assert 0 < d[13] < r["nodes"]                             # frame updates: pushes and pops
     # the product library, same cubes: same lengths, no diagnostics
     tb, _ = Tables.build("304", "block")
