@@ -1388,7 +1388,7 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
                 d_start = true;
 #endif
                 // ---- start the item: its prefix is replayed move by move through
-                // the probe step below, all six frames (kernels.cu:1552-1614) ----
+                // the probe step below, every frame (kernels.cu:1552-1614) ----
                 unsigned long long nb;
                 if (QUEUE) {
                     const bool banded = !SERVE && !spl && (chunk_left & CL_BANDED);
