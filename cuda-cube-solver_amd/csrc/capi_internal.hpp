@@ -45,6 +45,7 @@ inline int device_cus() {
 
 // Device allocations and events owned by one call: freed on every return path.
 struct DeviceBuffers {
+    static constexpr bool holds_keys = true;  // room for the item keys of a banded order
     std::vector<void*> ptrs;
     std::vector<hipEvent_t> events;
     template <typename T>

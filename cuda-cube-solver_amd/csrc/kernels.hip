@@ -632,16 +632,24 @@ __device__ __forceinline__ bool in_band(uint32_t key, uint32_t b, uint32_t lo, u
 // Finer chunks cost more in fetches than they save at a round's end (with 16
 // chunks per lane, serving fell from 140 to 87 cubes/s at 150 QPS).
 __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned long long last_nodes,
-                                                  uint32_t n_items) {
+                                                  uint32_t n_items, bool banded = false) {
     unsigned long long per_item = (last_nodes * 13u) / (n_items ? n_items : 1u) + 1u;
     // cubes admitted and not yet finished (u32 differences stay right across wraparound)
     unsigned long long live = A.serving ? (unsigned long long)((uint32_t)ld_c(&A.q->mb_head) - ld_c(&A.q->n_final))
                                         : (unsigned long long)(A.q->n_jobs - ld_c(&A.q->n_final));
-    unsigned long long want = A.n_lanes / (live ? live : 1ull);
-    want = want < 4096ull ? 4096ull : want;
+    const unsigned long long share = A.n_lanes / (live ? live : 1ull);  // the cube's lanes
+    const unsigned long long want = share < 4096ull ? 4096ull : share;
     const unsigned long long target = A.serving && live <= SERVE_DONATE_LIVE ? A.chunk_nodes << 6 : A.chunk_nodes;
+    // A banded round's chunks hold at most 32, 16 or 8 visits (about 2, 1 or
+    // 0.5 of their band's items).  Its first bands hold the heavy items; in
+    // chunks of 64 visits they left a cube's lanes waiting on a few long
+    // chunks at every round's end (lanes without work: 20 of 64 per step on
+    // 512 cubes, against 9 in move order).  The more lanes a cube has, the
+    // more that costs: 32 visits while a cube has at most 64 lanes, 16 up to
+    // 128, 8 beyond.
+    const uint32_t cap = !banded ? 6u : share <= 64ull ? 5u : share <= 128ull ? 4u : 3u;
     uint32_t lg = 0;
-    while (lg < 6 && (per_item << (lg + 1)) <= target && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
+    while (lg < cap && (per_item << (lg + 1)) <= target && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
     return lg;
 }
 
@@ -711,8 +719,9 @@ __device__ void close_round(const SearchArgs& A, uint32_t c, unsigned long long 
     bool banded = false;
     if (A.key_hist)
         banded = band_bounds(A.key_hist + (size_t)c * KEY_BINS, J,
-                             plen == t_plen(tk) && A.keys[plen] && nodes >= BAND_MIN_NODES_PER_ITEM * A.pre_n[plen]);
-    uint32_t clog = chunk_log_for(A, nodes, A.pre_n[plen] * (banded ? N_BANDS : 1u));
+                             plen == t_plen(tk) && A.keys[plen] && nodes >= BAND_MIN_NODES_PER_ITEM * A.pre_n[plen] &&
+                                 A.q->n_jobs - ld_c(&A.q->n_final) >= BAND_MIN_LIVE);
+    uint32_t clog = chunk_log_for(A, nodes, A.pre_n[plen] * (banded ? N_BANDS : 1u), banded);
     st_c(&J->nodes, 0ull);
     st_c(&J->bound, nb);
     if (A.serving) st_c(&J->credit, serve_credit(A, J));

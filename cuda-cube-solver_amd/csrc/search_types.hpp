@@ -24,10 +24,14 @@ constexpr int DIAG_WORDS = 30;
 // rounds with prefixes of at most KEY_MAX_PLEN moves, and a long round (at
 // least BAND_MIN_NODES_PER_ITEM nodes per item in the round before) whose
 // previous round had the same prefix length visits its items in N_BANDS bands
-// of rising key.  (On 512 cubes the banded order ran 18% slower; on 4,096,
-// 10% faster.)
+// of rising key, while at least BAND_MIN_LIVE cubes are unfinished.  (With
+// banded chunks of at most 8 to 32 visits, by the cube's share of the lanes:
+// 4,096 cubes 3-4% faster than in move order, 256 to 1,024 cubes 12-19%;
+// 48 to 96 cubes 2-8% slower, and C4's 320 hard cubes on d4 4% slower
+// unless the last 64 go in move order.)
 constexpr int KEY_MAX_PLEN = 5;
-constexpr int BAND_MIN_CUBES = 2048;
+constexpr int BAND_MIN_CUBES = 256;
+constexpr unsigned BAND_MIN_LIVE = 64;
 // ... and whose keys fit in this many bytes (1.24 MB per cube: about 13,500 cubes)
 constexpr unsigned long long KEY_MAX_BYTES = 16ull << 30;
 constexpr uint32_t N_BANDS = 16;  // a multiple of 4, at most 16

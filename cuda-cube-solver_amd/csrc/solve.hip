@@ -49,6 +49,7 @@ hipError_t zero_async(void* p, size_t bytes, hipStream_t s) {
 
 // Device memory from a fixed region, 256-byte aligned; nothing is allocated.
 struct Arena {
+    static constexpr bool holds_keys = false;  // the keys of a banded order need 1.24 MB per cube
     uint8_t* base = nullptr;
     size_t cap = 0, used = 0;
     template <typename T>
@@ -212,7 +213,7 @@ int solve_persistent(ccs_tables* t, const ccs_solve_opts_t& o, SearchArgs& A, in
     size_t key_total = 0;
     // (rows of whole 8-byte words: the search reads eight keys at a time)
     for (int p = o.prefix_len; p <= KEY_MAX_PLEN; ++p) key_total += 2ull * nl * ((t->prefix_n[p] + 7ull) / 8ull * 2ull);
-    if (nl >= BAND_MIN_CUBES && key_total && key_total * 4 <= KEY_MAX_BYTES) {
+    if (Mem::holds_keys && nl >= BAND_MIN_CUBES && key_total && key_total * 4 <= KEY_MAX_BYTES) {
         uint32_t *d_keys = nullptr, *d_hist = nullptr;
         HIP_TRY(mem.alloc(&d_keys, key_total * 4));
         HIP_TRY(mem.alloc(&d_hist, (size_t)nl * KEY_BINS * 4));

@@ -165,6 +165,19 @@ def test_c3_seed3_2048_banded_order(td4t, seeded):
     assert _solves(cubes, sols)
 
 
+def test_c3_seed3_256_banded_in_small_chunks(td4t, seeded):
+    """256 seed-3 states: banded rounds whose chunks hold at most 8 visits (a
+    cube has 768 of the launch's lanes), and move order again once fewer than
+    64 cubes are unfinished; the first 64 cubes give the oracle's lengths and
+    completed-round nodes, and every solution solves."""
+    cubes = states.uniform_states(256, 3)
+    sols = GpuSolver(table=td4t, mode="batched").solve(cubes)
+    fx = seeded["c3_d4t_seed3_first64"]
+    assert [len(s.moves) for s in sols[:64]] == fx["lengths"]
+    assert [_complete_nodes(s) for s in sols[:64]] == fx["complete_round_nodes"]
+    assert _solves(cubes, sols)
+
+
 def test_autotuner_on_a_small_sample(td4t, seeded):
     """The autotuner runs its whole grid on a small seeded sample (it raises if
     any setting changes an optimal length), keeps the fastest trial, and its
