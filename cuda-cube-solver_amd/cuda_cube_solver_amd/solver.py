@@ -185,7 +185,7 @@ class GpuSolver:
         self.max_bound = max_bound  # 0: no limit below the longest optimal solution (20)
         self.block = block  # lanes per workgroup (0: the largest the kernels are built for)
         self.probes_per_step = probes_per_step  # 1 or 2 table probes in flight per lane (0: 2)
-        self.read_mode = read_mode  # a step's two table reads: 1 paired, 2 in turn (0: by batch size)
+        self.read_mode = read_mode  # a step's two table reads: 1 paired, 2 in turn (0: by form and batch size)
         self.last_stats: _lib.SolveStats | None = None
 
     def _opts(self, mode: str | None = None) -> _lib.SolveOpts:

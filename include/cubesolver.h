@@ -141,7 +141,8 @@ typedef struct {
                                inverse frame (2) or the top remaining frame (1); nodes are the same */
     int32_t read_mode;       /* a step's two table reads: 1 paired (issued together; a lane with one
                                probe reads its block twice), 2 in turn (each read waited for before the
-                               next), 0: paired for batches of at least 1,024 cubes, else in turn */
+                               next), 0: paired in the plain form and for persistent batches of at
+                               least 1,024 cubes, else in turn */
 } ccs_solve_opts_t;
 
 typedef struct {
