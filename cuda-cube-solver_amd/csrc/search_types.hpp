@@ -27,8 +27,8 @@ constexpr int DIAG_WORDS = 30;
 // of rising key, while at least BAND_MIN_LIVE cubes are unfinished.  (With
 // banded chunks of at most 8 to 32 visits, by the cube's share of the lanes:
 // 4,096 cubes 3-4% faster than in move order, 256 to 1,024 cubes 12-19%;
-// 48 to 96 cubes 2-8% slower, and C4's 320 hard cubes on d4 4% slower
-// unless the last 64 go in move order.)
+// 48 to 96 cubes 2-8% slower, and 320 hard cubes on d4 (length 18 or more,
+// and flip-heavy) 4% slower unless the last 64 go in move order.)
 constexpr int KEY_MAX_PLEN = 5;
 constexpr int BAND_MIN_CUBES = 256;
 constexpr unsigned BAND_MIN_LIVE = 64;
