@@ -627,10 +627,13 @@ __device__ __forceinline__ bool in_band(uint32_t key, uint32_t b, uint32_t lo, u
 }
 // Items per fetch for a round: about chunk_nodes nodes per chunk, estimated from
 // the previous round's nodes and the usual growth of ~13x per bound, but at
-// least enough chunks for this cube's share of the launch's lanes (and never
-// fewer than 4096), so that the last cubes of a batch still fill the GPU.
-// Finer chunks cost more in fetches than they save at a round's end (with 16
-// chunks per lane, serving fell from 140 to 87 cubes/s at 150 QPS).
+// least 1.5 chunks per lane of this cube's share of the launch's lanes (and
+// never fewer than 256), so that the last cubes of a batch still fill the GPU.
+// Finer chunks cost more in fetches than they save at a round's end: with at
+// least 4096 chunks per round, a 4,096-cube batch fetched 4-move rounds four
+// items at a time and ran at 415 against 564 cubes/s (256 cubes: 192 against
+// 242); with 16 chunks per lane, serving fell from 140 to 87 cubes/s at 150
+// QPS.
 __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned long long last_nodes,
                                                   uint32_t n_items, bool banded = false) {
     unsigned long long per_item = (last_nodes * 13u) / (n_items ? n_items : 1u) + 1u;
@@ -638,7 +641,7 @@ __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned 
     unsigned long long live = A.serving ? (unsigned long long)((uint32_t)ld_c(&A.q->mb_head) - ld_c(&A.q->n_final))
                                         : (unsigned long long)(A.q->n_jobs - ld_c(&A.q->n_final));
     const unsigned long long share = A.n_lanes / (live ? live : 1ull);  // the cube's lanes
-    const unsigned long long want = share < 4096ull ? 4096ull : share;
+    const unsigned long long want = share + share / 2 < 256ull ? 256ull : share + share / 2;
     const unsigned long long target = A.serving && live <= SERVE_DONATE_LIVE ? A.chunk_nodes << 6 : A.chunk_nodes;
     // A banded round's chunks hold at most 32, 16 or 8 visits (about 2, 1 or
     // 0.5 of their band's items).  Its first bands hold the heavy items; in

@@ -189,9 +189,10 @@ int solve_persistent(ccs_tables* t, const ccs_solve_opts_t& o, SearchArgs& A, in
         lj[k] = jobs[live[k]];
         uint32_t b = lj[k].bound;
         uint32_t plen = std::min(std::min(b, (uint32_t)o.prefix_len), (uint32_t)MAX_PLEN);
-        // first round: chunks of up to 64 items, and at least max(4096, the
-        // cube's share of the lanes) chunks per round
-        const unsigned long long want = std::max<unsigned long long>(4096ull, A.n_lanes / (unsigned long long)nl);
+        // first round: chunks of up to 64 items, and at least max(256, 1.5 x the
+        // cube's share of the lanes) chunks per round, as the device's later rounds
+        const unsigned long long share = A.n_lanes / (unsigned long long)nl;
+        const unsigned long long want = std::max<unsigned long long>(256ull, share + share / 2);
         unsigned long long clog = 0;
         while (clog < 6 && (unsigned long long)(t->prefix_n[plen] >> (clog + 1)) >= want) ++clog;
         lj[k].ticket = ((unsigned long long)b << 49) | ((unsigned long long)plen << 54) | (clog << 59);
