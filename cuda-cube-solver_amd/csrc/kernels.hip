@@ -598,7 +598,7 @@ __device__ __forceinline__ uint32_t next_item(uint32_t item, uint32_t left, uint
     if (left & (blk - 1u)) return item + 1u;  // within a block
     const uint32_t q = (n + (1u << lg) - 1u) >> lg;
     return item + (q - 1u) * blk + 1u;        // first item of the chunk's next block
-}                     // log2 of the items a lane takes per fetch (0..6)
+}
 
 __device__ __forceinline__ uint32_t t_next(unsigned long long t) { return (uint32_t)(t & T_NEXT_MASK); }
 __device__ __forceinline__ uint32_t t_inflight(unsigned long long t) { return (uint32_t)((t & T_INF_MASK) >> 28); }
@@ -649,10 +649,11 @@ __device__ __forceinline__ uint32_t chunk_log_for(const SearchArgs& A, unsigned 
     // chunks at every round's end (lanes without work: 20 of 64 per step on
     // 512 cubes, against 9 in move order).  The more lanes a cube has, the
     // more that costs: 32 visits while a cube has at most 64 lanes, 16 up to
-    // 128, 8 beyond.  Rounds in move order take at most 64 items per chunk
-    // while a cube has at most 512 lanes, 32 up to 2,048, 16 beyond (small
-    // batches and the last cubes of a large one: 0-3% faster).
-    const uint32_t cap = !banded ? (share <= 512ull ? 6u : share <= 2048ull ? 5u : 4u)
+    // 128, 8 beyond.  Rounds in move order take at most 128 items per chunk
+    // while a cube has at most 512 lanes, 32 up to 2,048, 16 beyond (4,096
+    // cubes: 575-591 against 555-564 cubes/s with 64 items; small batches
+    // and the last cubes of a large one: 0-3% faster with 32 or 16).
+    const uint32_t cap = !banded ? (share <= 512ull ? 7u : share <= 2048ull ? 5u : 4u)
                                  : share <= 64ull ? 5u : share <= 128ull ? 4u : 3u;
     uint32_t lg = 0;
     while (lg < cap && (per_item << (lg + 1)) <= target && (unsigned long long)(n_items >> (lg + 1)) >= want) ++lg;
