@@ -37,7 +37,7 @@ BYTES_PER_PROBE = 32  # algorithmic: one 32-byte table block per lookup
 # (FETCH_SIZE x2: it books each 128-byte read request as 64 B; WRITE_SIZE x1):
 # profiles/search_kernel_d4t_pmc.md.  A random 32-byte block costs one 128-byte line.
 FETCHED_BYTES_PER_PROBE = 128.1
-WRITTEN_BYTES_PER_PROBE = 0.061
+WRITTEN_BYTES_PER_PROBE = 0.087
 
 
 def log(msg: str) -> None:
