@@ -87,21 +87,24 @@ def build_summary(lines: list[str], counts: list[int], total_s: float) -> dict:
 
 
 def gather_ceiling(tb) -> dict:
-    """Random-probe ceiling of the memory system on this table: 2^32 reads of a
-    random 32-byte block's minimum word plus one offset word (the search's
-    access pattern, four independent probes per lane), and a streaming read of
-    the whole table, timed with HIP events in the library."""
+    """Random-probe ceiling of the memory system on this table: 2^32 probes of
+    random 32-byte blocks, four independent probes per lane, read as the block
+    layout's probe reads (the minimum word plus one offset word) and as the
+    search reads its half-block copy (one 16-byte load); and a streaming read
+    of the whole table.  Timed with HIP events in the library."""
     import ctypes
 
     from cuda_cube_solver_amd import _lib
 
     out = {}
-    for mode, n in ((0, 1 << 32), (2, 0)):
+    for mode, n in ((0, 1 << 32), (3, 1 << 32), (2, 0)):
         ms, cs = ctypes.c_double(), ctypes.c_uint64()
         _lib.check(_lib.lib().ccs_gather_bench(tb.handle, mode, n, 16, ctypes.byref(ms), ctypes.byref(cs)))
         if mode == 0:
             out["random_probes_g_per_s"] = round(n / ms.value / 1e6, 2)
             out["random_probe_algorithmic_gb_s"] = round(n * BYTES_PER_PROBE / ms.value / 1e6, 1)
+        elif mode == 3:
+            out["random_probes_one_load_g_per_s"] = round(n / ms.value / 1e6, 2)
         else:
             out["stream_gb_s"] = round(tb.nbytes("block") / ms.value / 1e6, 1)
     return out

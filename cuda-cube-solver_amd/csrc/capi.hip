@@ -492,7 +492,7 @@ int ccs_unrank_check(ccs_tables_t* t, const uint64_t* idx, int n, uint64_t* out)
 
 int ccs_gather_bench(ccs_tables_t* t, int mode, uint64_t n_probes, int blocks_per_cu, double* ms,
                      uint64_t* checksum) {
-    if (!t || mode < 0 || mode > 2 || !ms) return fail(CCS_EINVAL, "bad arguments");
+    if (!t || mode < 0 || mode > 3 || !ms) return fail(CCS_EINVAL, "bad arguments");
     int fmt = t->format < 0 ? 0 : t->format;
     if (!t->d_table[fmt]) return fail(CCS_EINVAL, "no table resident");
     DeviceBuffers buf;

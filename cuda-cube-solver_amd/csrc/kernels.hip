@@ -1675,11 +1675,13 @@ __global__ void __launch_bounds__(SEARCH_WG, SEARCH_WAVES_PER_SIMD) search_kerne
 }
 
 // ---------------------------------------------------------------------------
-// Gather micro-benchmark: random 32-byte table blocks read the way a probe
-// reads them (block minimum word + one offset word), to measure the random-
-// probe ceiling of the memory system; mode 2 streams the table for a
-// calibration of the fetch counters.  Four independent probes per thread per
-// iteration keep several misses in flight.
+// Gather micro-benchmark: random 32-byte table blocks read the way a probe of
+// the block layout reads them (block minimum word + one offset word; mode 1
+// the same with nontemporal loads), or one 16-byte half of a random block in
+// a single load, as the search reads its half-block layout (mode 3), to
+// measure the random-probe ceiling of the memory system; mode 2 streams the
+// table for a calibration of the fetch counters.  Four independent probes per
+// thread per iteration keep several misses in flight.
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -1707,8 +1709,14 @@ __global__ void __launch_bounds__(256) gather_kernel(const uint32_t* __restrict_
                 unsigned long long h = mix64(i + k + 0x9E3779B97F4A7C15ull);
                 const uint32_t* b = blk + (h % n_blocks) * 8u;
                 uint32_t w = (uint32_t)(h >> 61);
-                if (MODE == 1) v[k] = __builtin_nontemporal_load(b) + __builtin_nontemporal_load(b + w);
-                else v[k] = b[0] + b[w];
+                if (MODE == 3) {
+                    const uint4 q = reinterpret_cast<const uint4*>(b)[w & 1u];
+                    v[k] = q.x ^ q.y ^ q.z ^ q.w;
+                } else if (MODE == 1) {
+                    v[k] = __builtin_nontemporal_load(b) + __builtin_nontemporal_load(b + w);
+                } else {
+                    v[k] = b[0] + b[w];
+                }
             }
             acc += v[0] ^ v[1] ^ v[2] ^ v[3];
         }
@@ -1841,6 +1849,7 @@ hipError_t launch_gather(int mode, int grid, const uint32_t* blk, unsigned long 
                          unsigned long long n_probes, unsigned long long* sum, hipStream_t s) {
     if (mode == 0) hipLaunchKernelGGL(gather_kernel<0>, dim3(grid), dim3(256), 0, s, blk, n_blocks, n_probes, sum);
     else if (mode == 1) hipLaunchKernelGGL(gather_kernel<1>, dim3(grid), dim3(256), 0, s, blk, n_blocks, n_probes, sum);
+    else if (mode == 3) hipLaunchKernelGGL(gather_kernel<3>, dim3(grid), dim3(256), 0, s, blk, n_blocks, n_probes, sum);
     else hipLaunchKernelGGL(gather_kernel<2>, dim3(grid), dim3(256), 0, s, blk, n_blocks, n_probes, sum);
     return hipGetLastError();
 }

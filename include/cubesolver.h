@@ -245,9 +245,11 @@ int ccs_server_stop(ccs_server_t* s);
 /* ---- diagnostics ------------------------------------------------------- */
 
 /* Random-probe micro-benchmark on the resident solving table: n_probes
- * random 32-byte blocks read as a probe reads them (mode 0; mode 1 with
- * nontemporal loads), or one streaming pass over the table (mode 2).  The
- * measured rate is the memory system's ceiling for the search's probes.
+ * random 32-byte blocks read as a probe of the block layout reads them (two
+ * 4-byte reads: mode 0; mode 1 with nontemporal loads), one random 16-byte
+ * half-block in one load as the search reads its half-block copy (mode 3),
+ * or one streaming pass over the table (mode 2).  The measured rate is the
+ * memory system's ceiling for the search's probes.
  * Counterpart of the reference's bench/gpu_gather.py. */
 int ccs_gather_bench(ccs_tables_t* t, int mode, uint64_t n_probes, int blocks_per_cu, double* ms, uint64_t* checksum);
 

@@ -274,3 +274,15 @@ def test_server_stops_promptly_while_busy(td4t):
             again.submit(c, tag=i)
         got = again.collect(len(probe), timeout_s=120)
     assert [got[i].length for i in range(len(probe))] == [len(s.moves) for s in alone]
+
+
+def test_gather_bench_modes(td4t):
+    """The random-probe ceiling benchmark runs in every mode on the resident
+    d4t table (two 4-byte reads, the same nontemporal, one 16-byte half-block
+    load, a streaming pass) and refuses an unknown mode."""
+    for mode, n in ((0, 1 << 22), (1, 1 << 22), (3, 1 << 22), (2, 0)):
+        ms, cs = ctypes.c_double(), ctypes.c_uint64()
+        _lib.check(_lib.lib().ccs_gather_bench(td4t.handle, mode, n, 8, ctypes.byref(ms), ctypes.byref(cs)))
+        assert ms.value > 0, mode
+    ms, cs = ctypes.c_double(), ctypes.c_uint64()
+    assert _lib.lib().ccs_gather_bench(td4t.handle, 4, 1, 8, ctypes.byref(ms), ctypes.byref(cs)) == _lib.CCS_EINVAL
