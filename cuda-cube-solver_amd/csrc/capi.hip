@@ -229,7 +229,10 @@ int ccs_tables_open(const char* path, const char* kind, int format, ccs_tables_t
     }
     for (uint64_t off = 0, i = 0; off < want && result == CCS_OK; off += CH, ++i) {
         int b = (int)(i & 1);
-        if (i >= 2) hipEventSynchronize(ev[b]);
+        if (i >= 2 && hipEventSynchronize(ev[b]) != hipSuccess) {  // the buffer's previous upload
+            result = fail(CCS_EDEVICE, "table upload failed");
+            break;
+        }
         uint64_t n = std::min(CH, want - off);
         if (fread(pin[b], 1, n, fh) != n) {
             result = fail(CCS_EIO, std::string(path) + ": short read");
@@ -239,10 +242,11 @@ int ccs_tables_open(const char* path, const char* kind, int format, ccs_tables_t
             hipEventRecord(ev[b], t->stream) != hipSuccess)
             result = fail(CCS_EDEVICE, "table upload failed");
     }
-    hipStreamSynchronize(t->stream);
+    if (hipStreamSynchronize(t->stream) != hipSuccess && result == CCS_OK)
+        result = fail(CCS_EDEVICE, "table upload failed");
     for (int i = 0; i < 2; ++i) {
-        if (pin[i]) hipHostFree(pin[i]);
-        if (ev[i]) hipEventDestroy(ev[i]);
+        if (pin[i]) (void)hipHostFree(pin[i]);
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
     }
     fclose(fh);
     if (result != CCS_OK) return result;
@@ -431,7 +435,7 @@ int ccs_tables_save(ccs_tables_t* t, int format, const char* path) {
             break;
         }
     }
-    hipHostFree(pin);
+    (void)hipHostFree(pin);
     if (fclose(fh) != 0 && result == CCS_OK) result = fail(CCS_EIO, "close " + tmp);
     if (result != CCS_OK) {
         remove(tmp.c_str());

@@ -109,18 +109,19 @@ struct ccs_tables {
     uint32_t prefix_n[ccs::MAX_PLEN + 1] = {};
     hipStream_t stream = nullptr;
 
+    // Frees everything; errors are ignored here (nothing is left to report them to).
     ~ccs_tables() {
-        if (stream) hipStreamSynchronize(stream);
-        hipFree(d_lut);
-        hipFree(d_classidx);
-        hipFree(d_class_rep);
-        hipFree(d_class_stab);
-        hipFree(d_class_lut);
-        for (auto* p : d_table) hipFree(p);
-        hipFree(d_halves);
-        hipFree(d_depths);
-        for (auto* p : d_prefix) hipFree(p);
-        if (stream) hipStreamDestroy(stream);
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipFree(d_lut);
+        (void)hipFree(d_classidx);
+        (void)hipFree(d_class_rep);
+        (void)hipFree(d_class_stab);
+        (void)hipFree(d_class_lut);
+        for (auto* p : d_table) (void)hipFree(p);
+        (void)hipFree(d_halves);
+        (void)hipFree(d_depths);
+        for (auto* p : d_prefix) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
