@@ -236,3 +236,17 @@ def test_cli_unknown_table(tmp_path):
     out = subprocess.run([sys.executable, "-m", "cuda_cube_solver_amd", "solve", "--table", "nope"],
                          input="R U\n", capture_output=True, text=True, env=env)
     assert out.returncode != 0 and "unknown table 'nope'" in out.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with a message naming the
+    single-GPU scope, before it touches a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parent.parent
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
